@@ -1,0 +1,154 @@
+"""cother_amd: MI355X-native COTHER search hot path (Python bindings).
+
+The product is the C-ABI library ``libcother_hip.so`` built in-tree for
+gfx950 (see include/cother_hip.h); this module binds it with ctypes for the
+tests and the benchmark.  Loading fails loudly when the library is missing:
+there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libcother_hip.so")
+ENVDIR = os.path.join(HERE, "env")
+
+NAA, NCV, NSS, NTRN, DDSTRIDE = 20, 19, 3, 7, 128
+
+
+class CthPset(ctypes.Structure):
+    """Mirror of struct cth_pset (cother_amd/csrc/cth_pset.h)."""
+    _fields_ = [
+        ("nprof", ctypes.c_int32),
+        ("npos", ctypes.c_int64),
+        ("len", ctypes.POINTER(ctypes.c_int32)),
+        ("dist", ctypes.POINTER(ctypes.c_int64)),
+        ("eno", ctypes.POINTER(ctypes.c_float)),
+        ("bkg", ctypes.POINTER(ctypes.c_float)),
+        ("aa", ctypes.POINTER(ctypes.c_char)),
+        ("sss", ctypes.POINTER(ctypes.c_char)),
+        ("trg", ctypes.POINTER(ctypes.c_float)),
+        ("trn", ctypes.POINTER(ctypes.c_float)),
+        ("ss", ctypes.POINTER(ctypes.c_float)),
+        ("cv", ctypes.POINTER(ctypes.c_float)),
+        ("cvprior", ctypes.POINTER(ctypes.c_float)),
+        ("cvnorm2", ctypes.POINTER(ctypes.c_float)),
+        ("dd", ctypes.POINTER(ctypes.c_uint8)),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load libcother_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise RuntimeError(f"{LIBPATH} not built; run python -m cother_amd.build or __graft_entry__.build()")
+    L = ctypes.CDLL(LIBPATH)
+    P = ctypes.POINTER(CthPset)
+    L.cth_synth_make.restype = P
+    L.cth_synth_make.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                 ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_int]
+    L.cth_pset_free.argtypes = [P]
+    L.cth_pset_alloc.restype = P
+    L.cth_pset_alloc.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int32)]
+    L.cth_last_error.restype = ctypes.c_char_p
+    L.cth_ctx_create.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]
+    L.cth_ctx_destroy.argtypes = [ctypes.c_void_p]
+    L.cth_upload_queries.argtypes = [ctypes.c_void_p, P]
+    L.cth_upload_db.argtypes = [ctypes.c_void_p, P]
+    L.cth_score_block.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_void_p, ctypes.c_void_p]
+    _lib = L
+    return L
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed: {lib().cth_last_error().decode()}")
+
+
+class Pset:
+    """A host profile set owned by the library (cth_pset)."""
+
+    def __init__(self, ptr):
+        if not ptr:
+            raise MemoryError("profile set allocation failed")
+        self.ptr = ptr
+        self.s = ptr.contents
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().cth_pset_free(self.ptr)
+            self.ptr = None
+
+    @property
+    def nprof(self):
+        return self.s.nprof
+
+    @property
+    def npos(self):
+        return self.s.npos
+
+    def lens(self):
+        return np.ctypeslib.as_array(self.s.len, (self.nprof,)).copy()
+
+    def dists(self):
+        return np.ctypeslib.as_array(self.s.dist, (self.nprof,)).copy()
+
+    def field(self, name):
+        n, p = self.npos, self.nprof
+        shapes = {
+            "trg": (NAA, n), "ss": (NSS, n), "cv": (NCV, n), "cvprior": (n,), "cvnorm2": (n,),
+            "trn": (NTRN, n + p), "bkg": (p, NAA), "eno": (p,), "dd": (n, DDSTRIDE),
+        }
+        if name in ("aa", "sss"):
+            return np.ctypeslib.as_array(ctypes.cast(getattr(self.s, name), ctypes.POINTER(ctypes.c_uint8)), (n,))
+        return np.ctypeslib.as_array(getattr(self.s, name), shapes[name])
+
+
+def synth(nprof, len_mode, median=0.0, sigma=0.0, lmin=0, lmax=0, dist_mode=1, sort_desc=1, seed=1, threads=None):
+    """Seeded synthetic profile set (see cother_amd/csrc/synth.cpp)."""
+    if threads is None:
+        threads = min(16, os.cpu_count() or 1)
+    return Pset(lib().cth_synth_make(nprof, len_mode, median, sigma, lmin, lmax, dist_mode, sort_desc, seed,
+                                     threads))
+
+
+class Context:
+    """A device context (cth_ctx): tables, uploaded queries, resident DB."""
+
+    def __init__(self, device=0, envdir=ENVDIR):
+        L = lib()
+        h = ctypes.c_void_p()
+        _check(L.cth_ctx_create(device, envdir.encode(), ctypes.byref(h)), "cth_ctx_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().cth_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def upload_queries(self, qs):
+        _check(lib().cth_upload_queries(self.h, qs.ptr), "cth_upload_queries")
+        self.qs = qs
+
+    def upload_db(self, db):
+        _check(lib().cth_upload_db(self.h, db.ptr), "cth_upload_db")
+        self.db = db
+
+    def score_block(self, qidx, p0, p1):
+        lq = int(self.qs.lens()[qidx])
+        d, ln = self.db.dists(), self.db.lens()
+        ncols = int(d[p1 - 1] + ln[p1 - 1] - d[p0])
+        sc = np.empty((lq, ncols), np.float32)
+        md = np.empty((lq, ncols), np.float32)
+        _check(lib().cth_score_block(self.h, qidx, p0, p1, sc.ctypes.data, md.ctypes.data), "cth_score_block")
+        return sc, md

@@ -1,0 +1,129 @@
+// capi.hip: the C-ABI of libcother_hip (declared in include/cother_hip.h).
+//
+// A context owns one device, one HIP stream, the score tables, the uploaded
+// query set and the resident template database, plus scratch buffers that
+// grow on demand.  Errors are reported as negative return codes with a
+// message available from cth_last_error(); nothing throws across the ABI.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+
+#include "cth_ctx.h"
+
+namespace {
+thread_local char g_err[1024] = "";
+}
+
+void cth_set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+}
+
+extern "C" const char* cth_last_error(void) { return g_err; }
+
+extern "C" int cth_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int cth_ctx_reserve(cth_ctx* c, void** p, size_t* cap, size_t need) {
+  if (need <= *cap) return 0;
+  if (*p) CTH_HIP_CHECK(hipFree(*p));
+  *p = nullptr;
+  *cap = 0;
+  size_t n = need + need / 8 + 4096;
+  CTH_HIP_CHECK(hipMalloc(p, n));
+  *cap = n;
+  (void)c;
+  return 0;
+}
+
+extern "C" int cth_ctx_create(int device, const char* envdir, cth_ctx** out) {
+  *out = nullptr;
+  int ndev = 0;
+  CTH_HIP_CHECK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) { cth_set_error("device %d not present (%d devices)", device, ndev); return -1; }
+  CTH_HIP_CHECK(hipSetDevice(device));
+  cth_ctx* c = (cth_ctx*)calloc(1, sizeof(cth_ctx));
+  if (!c) { cth_set_error("out of host memory"); return -1; }
+  c->device = device;
+  int rc = cth_tables_load(envdir, &c->htab);
+  if (rc) { cth_set_error("cannot load score tables from %s (code %d)", envdir, rc); free(c); return -1; }
+  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+    cth_set_error("hipStreamCreate failed"); free(c); return -1;
+  }
+  if (hipMalloc((void**)&c->dtab, sizeof(cth_tables)) != hipSuccess ||
+      hipMemcpy(c->dtab, &c->htab, sizeof(cth_tables), hipMemcpyHostToDevice) != hipSuccess) {
+    cth_set_error("cannot upload score tables"); cth_ctx_destroy(c); return -1;
+  }
+  *out = c;
+  return 0;
+}
+
+extern "C" void cth_ctx_destroy(cth_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->st) (void)hipStreamSynchronize(c->st);
+  cth_devset_free(&c->q);
+  cth_devset_free(&c->db);
+  void* bufs[] = {c->dtab, c->scores, c->mods, c->btck, c->dpout, c->work, c->mapbuf};
+  for (void* p : bufs)
+    if (p) (void)hipFree(p);
+  if (c->h_pin) (void)hipHostFree(c->h_pin);
+  if (c->st) (void)hipStreamDestroy(c->st);
+  free(c);
+}
+
+extern "C" int cth_upload_queries(cth_ctx* c, const cth_pset* qs) {
+  CTH_HIP_CHECK(hipSetDevice(c->device));
+  cth_devset_free(&c->q);
+  c->has_q = 0;
+  if (cth_devset_upload(qs, &c->q)) return -1;
+  c->has_q = 1;
+  return 0;
+}
+
+extern "C" int cth_upload_db(cth_ctx* c, const cth_pset* db) {
+  CTH_HIP_CHECK(hipSetDevice(c->device));
+  cth_devset_free(&c->db);
+  c->has_db = 0;
+  if (cth_devset_upload(db, &c->db)) return -1;
+  c->has_db = 1;
+  return 0;
+}
+
+extern "C" int cth_launch_score(const cth_dpset* q, int qp, const cth_dpset* db, int64_t col0, int ncols,
+                                int64_t ldsc, const cth_tables* dtabs, int lq, float* scores, float* mods,
+                                hipStream_t st);
+
+// Score matrix of query qidx against template profiles [p0,p1), copied to
+// host arrays [Lq][ncols] (ncols = positions of those profiles).
+extern "C" int cth_score_block(cth_ctx* c, int qidx, int p0, int p1, float* h_scores, float* h_mods) {
+  if (!c->has_q || !c->has_db) { cth_set_error("score_block: queries or database not uploaded"); return -1; }
+  if (qidx < 0 || qidx >= c->q.d.nprof) { cth_set_error("score_block: query %d out of range", qidx); return -1; }
+  if (p0 < 0 || p1 > c->db.d.nprof || p0 >= p1) { cth_set_error("score_block: bad profile range [%d,%d)", p0, p1); return -1; }
+  CTH_HIP_CHECK(hipSetDevice(c->device));
+  const int lq = c->q.h_len[qidx];
+  const int64_t col0 = c->db.h_dist[p0];
+  const int64_t ncols = c->db.h_dist[p1 - 1] + c->db.h_len[p1 - 1] - col0;
+  if (ncols > (1ll << 30)) { cth_set_error("score_block: too many columns"); return -1; }
+  const int64_t ldsc = (ncols + 63) / 64 * 64;
+  const size_t need = sizeof(float) * (size_t)lq * ldsc;
+  if (cth_ctx_reserve(c, (void**)&c->scores, &c->cap_scores, need)) return -1;
+  if (cth_ctx_reserve(c, (void**)&c->mods, &c->cap_mods, need)) return -1;
+  if (cth_launch_score(&c->q.d, qidx, &c->db.d, col0, (int)ncols, ldsc, c->dtab, lq, c->scores, c->mods, c->st))
+    return -1;
+  CTH_HIP_CHECK(hipStreamSynchronize(c->st));
+  CTH_HIP_CHECK(hipMemcpy2D(h_scores, sizeof(float) * ncols, c->scores, sizeof(float) * ldsc, sizeof(float) * ncols,
+                            lq, hipMemcpyDeviceToHost));
+  CTH_HIP_CHECK(hipMemcpy2D(h_mods, sizeof(float) * ncols, c->mods, sizeof(float) * ldsc, sizeof(float) * ncols,
+                            lq, hipMemcpyDeviceToHost));
+  return 0;
+}

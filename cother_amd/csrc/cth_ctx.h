@@ -1,0 +1,28 @@
+// cth_ctx.h: the device context behind the C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "cth_dev.h"
+#include "cth_devdb.h"
+#include "cth_tables.h"
+
+struct cth_ctx {
+  int device;
+  hipStream_t st;
+  cth_tables htab;
+  cth_tables* dtab;
+  cth_devset q, db;
+  int has_q, has_db;
+  // scratch, grown on demand
+  float* scores; size_t cap_scores;
+  float* mods; size_t cap_mods;
+  uint8_t* btck; size_t cap_btck;
+  float* dpout; size_t cap_dpout;
+  void* work; size_t cap_work;
+  void* mapbuf; size_t cap_mapbuf;
+  void* h_pin; size_t cap_pin;
+};
+
+int cth_ctx_reserve(cth_ctx* c, void** p, size_t* cap, size_t need);
+extern "C" void cth_ctx_destroy(cth_ctx* c);

@@ -1,0 +1,81 @@
+/* cother_hip.h: C-ABI of libcother_hip, the MI355X (gfx950) implementation
+ * of COTHER's search hot path.
+ *
+ * The library replaces the device side of the reference's batch-processing
+ * interface, class CuBatchProcessing (src/libmycu/cupro/CuBatchProcessing.cuh:
+ * 47-244), together with the device-memory manager it drives
+ * (src/libmycu/cupro/CuDeviceMemory.cuh:47-770).  Plain pointers and sizes
+ * only; every entry point returns 0 on success and a negative code on error,
+ * with a message from cth_last_error() (the reference throws
+ * myruntime_error through MYCUDACHECK, src/libmycu/cucom/cucommon.h:22-55,
+ * and DevCommThread turns it into THREAD_MSG_ERROR,
+ * src/libmycu/cualn/DevCommThread.cpp:259-276).
+ *
+ * Calls on one context are serial (as calls into one CuBatchProcessing
+ * instance are, DevCommThread.cpp:296-356); distinct contexts may share a
+ * device.
+ */
+#ifndef COTHER_HIP_H
+#define COTHER_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- profile sets -----------------------------------------------------------
+ * Columnar host profile set; field conventions of the reference's packed
+ * profile model (src/libmycu/cupro/PM2DVectorFields.h:53-87) as filled by its
+ * profile readers (src/libmycu/cupro/IOProfileModelTxt.cu:296-1030).  The
+ * definition is in cother_amd/csrc/cth_pset.h. */
+typedef struct cth_pset cth_pset;
+
+/* Allocate an empty set of nprof profiles with the given lengths (positions
+ * laid out consecutively).  Replaces the reference's PMBatchProData buffers
+ * (src/libmycu/cupro/PMBatchProData.h). */
+cth_pset* cth_pset_alloc(int nprof, const int32_t* lens);
+void cth_pset_free(cth_pset* ps);
+
+/* Seeded synthetic profile set in the reference's value conventions (no
+ * reference counterpart: the reference has no generator; values follow the
+ * text profile round trip of src/libpro/srcpro/PMProfileModel.cpp:1856-2082
+ * and the adddist list transform of src/adddist/adddist.cpp:560-651).
+ * len_mode 0 uniform [lmin,lmax], 1 lognormal(median,sigma) clipped, 2 fixed;
+ * dist_mode 0 predicted maps, 1 C-alpha chains, 2 none; sort_desc orders by
+ * length, longest first (makedb order, src/libpro/srcpro/Db.cpp:606). */
+cth_pset* cth_synth_make(int nprof, int len_mode, double median, double sigma, int lmin, int lmax,
+                         int dist_mode, int sort_desc, uint64_t seed, int nthreads);
+
+/* ---- context ------------------------------------------------------------------ */
+typedef struct cth_ctx cth_ctx;
+
+/* Create a context on a device and load the score tables (ssss, cvs2s) from
+ * envdir.  Replaces the CuDeviceMemory constructor and its Cache*ScoresDevice
+ * calls (src/libmycu/cupro/CuDeviceMemory.cu:100-160,644-830). */
+int cth_ctx_create(int device, const char* envdir, cth_ctx** out);
+void cth_ctx_destroy(cth_ctx* c);
+const char* cth_last_error(void);
+/* Number of visible HIP devices (0 when none; the reference's
+ * Devices::RegisterDevices, src/libmycu/cualn/Devices.cu:59-108). */
+int cth_device_count(void);
+
+/* Upload the query set (CuDeviceMemory::CacheCompleteData query part,
+ * src/libmycu/cualn/JobDispatcher.cpp:325-355). */
+int cth_upload_queries(cth_ctx* c, const cth_pset* qs);
+/* Make a template database resident in HBM (replaces the per-chunk
+ * CuBatchProcessing::TransferCPMData, src/libmycu/cupro/CuBatchProcessing.cuh:
+ * 60-70, with a one-time upload). */
+int cth_upload_db(cth_ctx* c, const cth_pset* db);
+
+/* Score matrix of query qidx against template profiles [p0,p1), copied to
+ * host row-major [Lq][ncols] (ncols = positions of those profiles).
+ * The stage CuBatchScoreMatrix::ComputeScoreMatrixDevice
+ * (src/libmycu/cusco/CuBatchScoreMatrix.cu:122-404) computes on the device;
+ * this entry exposes its outputs (scores, modscores) for inspection. */
+int cth_score_block(cth_ctx* c, int qidx, int p0, int p1, float* scores, float* modscores);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,84 @@
+/* oracle.h: CPU restatement of the COTHER search hot path (TEST
+ * INFRASTRUCTURE ONLY).
+ *
+ * This library is the checker for the HIP implementation in cother_amd/.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it, and only to compute expected values or the timed CPU baseline;
+ * the product path never links or calls it.
+ *
+ * Each function restates the reference's algorithm (minmarg/cother 2.03.03,
+ * src/libmycu) in its single-precision operation order; the citation next to
+ * each function names the reference file:line it follows.  Arithmetic policy,
+ * shared with the HIP path and stated once here:
+ *   - IEEE binary32, round to nearest, no contraction except where the
+ *     reference source has the form a*b+c (or c+a*b, c-a*b) inside one
+ *     expression, which nvcc's default --fmad=true fuses; those are written as
+ *     explicit fmaf() calls;
+ *   - CUDA's __logf/__fdividef (approximate, NVIDIA-specific) are replaced by
+ *     orc_logf (a project-written logarithm, <= 1 ulp, defined below) and
+ *     IEEE division.  Their NVIDIA values are not reproduced: that part of the
+ *     parity is unpinned and documented as such.
+ */
+#ifndef COTHER_ORACLE_H
+#define COTHER_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* profile set: same field set and layout as the product's columnar profile
+ * set (reference field list cupro/PM2DVectorFields.h:53-87) */
+typedef struct orc_pset {
+  int32_t  nprof;
+  int64_t  npos;
+  int32_t* len;
+  int64_t* dist;
+  float*   eno;
+  float*   bkg;      /* [nprof*20] */
+  char*    aa;       /* [npos] */
+  char*    sss;      /* [npos] */
+  float*   trg;      /* [20*npos] */
+  float*   trn;      /* [7*(npos+nprof)] */
+  float*   ss;       /* [3*npos] */
+  float*   cv;       /* [19*npos] */
+  float*   cvprior;  /* [npos] */
+  float*   cvnorm2;  /* [npos] */
+  uint8_t* dd;       /* [npos*128]: n, d1..dn, 255 padding */
+} orc_pset;
+
+/* score tables read from the reference's parameter files */
+typedef struct orc_tables {
+  float ss[467];      /* serialized SS table: 2 header cells + 30x30 lower triangle (SerializedScoresSM.cuh:103-120) */
+  int   cvcard, cvshift;
+  float cvkeys[128], cvvals[128];
+  float cvkappa0inv, cvpowernu0, cvcterm;  /* libpro/srcpro/TCTXVECT.cpp:26-60 */
+  float dms[22 * 22]; /* pair score table, index min(d,20) with 255 -> 21 */
+} orc_tables;
+
+float orc_logf(float x);
+int   orc_tables_load(const char* envdir, orc_tables* t);
+
+/* single-cell terms (query position qi of profile set q, template position tj
+ * of profile set db; tp = template profile index owning tj) */
+float orc_cell_init(const orc_pset* q, int64_t qi, int qp, const orc_pset* db, int64_t tj, int tp);
+float orc_cell_ss(const orc_tables* t, const orc_pset* q, int64_t qi, const orc_pset* db, int64_t tj);
+float orc_cell_cvs(const orc_tables* t, const orc_pset* q, int64_t qi, const orc_pset* db, int64_t tj);
+float orc_cell_ddms(const orc_tables* t, const uint8_t* qrec, const uint8_t* trec);
+/* DDMS with segment bookkeeping exposed: m[3] top segment scores, c[3] packed coords */
+float orc_cell_ddms_ex(const orc_tables* t, const uint8_t* qrec, const uint8_t* trec, float m[3], uint32_t c[3]);
+
+/* DDMS with an explicit 22x22 pair-score table (index 21 = missing) */
+float orc_ddms_tab(const float* tab22, const uint8_t* qrec, const uint8_t* trec, float m[3], uint32_t c[3]);
+
+/* full score matrix of query profile qp against template profiles [p0,p1):
+ * scores/modscores are [Lq][ncols], ncols = positions of profiles p0..p1-1,
+ * column 0 = first position of profile p0 */
+void orc_score_matrix(const orc_tables* t, const orc_pset* q, int qp, const orc_pset* db, int p0, int p1,
+                      float* scores, float* modscores, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,102 @@
+"""ctypes bindings of the CPU oracle (oracle/liboracle.so) for the tests.
+
+The oracle is test infrastructure: it computes the expected values the HIP
+path is checked against.  Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg load it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ODIR = os.path.join(ROOT, "oracle")
+OLIB = os.path.join(ODIR, "liboracle.so")
+REFLIB = os.path.join(ODIR, "_ref", "libddms_ref.so")
+
+
+class OrcTables(ctypes.Structure):
+    _fields_ = [
+        ("ss", ctypes.c_float * 467),
+        ("cvcard", ctypes.c_int),
+        ("cvshift", ctypes.c_int),
+        ("cvkeys", ctypes.c_float * 128),
+        ("cvvals", ctypes.c_float * 128),
+        ("cvkappa0inv", ctypes.c_float),
+        ("cvpowernu0", ctypes.c_float),
+        ("cvcterm", ctypes.c_float),
+        ("dms", ctypes.c_float * (22 * 22)),
+    ]
+
+
+class Oracle:
+    def __init__(self, L):
+        self.L = L
+        vp = ctypes.c_void_p
+        L.orc_logf.restype = ctypes.c_float
+        L.orc_logf.argtypes = [ctypes.c_float]
+        L.orc_tables_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(OrcTables)]
+        L.orc_cell_init.restype = ctypes.c_float
+        L.orc_cell_init.argtypes = [vp, ctypes.c_int64, ctypes.c_int, vp, ctypes.c_int64, ctypes.c_int]
+        for f in ("orc_cell_ss", "orc_cell_cvs"):
+            getattr(L, f).restype = ctypes.c_float
+            getattr(L, f).argtypes = [ctypes.POINTER(OrcTables), vp, ctypes.c_int64, vp, ctypes.c_int64]
+        L.orc_cell_ddms.restype = ctypes.c_float
+        L.orc_cell_ddms.argtypes = [ctypes.POINTER(OrcTables), vp, vp]
+        L.orc_ddms_tab.restype = ctypes.c_float
+        L.orc_ddms_tab.argtypes = [vp, vp, vp, vp, vp]
+        L.orc_score_matrix.argtypes = [ctypes.POINTER(OrcTables), vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int,
+                                       vp, vp, ctypes.c_int]
+        self.tables = OrcTables()
+        from cother_amd import ENVDIR
+        rc = L.orc_tables_load(ENVDIR.encode(), ctypes.byref(self.tables))
+        if rc:
+            raise RuntimeError(f"oracle tables failed: {rc}")
+
+    def logf(self, x):
+        return self.L.orc_logf(x)
+
+    def score_matrix(self, q, qp, db, p0, p1, threads=None):
+        lq = int(q.lens()[qp])
+        d, ln = db.dists(), db.lens()
+        ncols = int(d[p1 - 1] + ln[p1 - 1] - d[p0])
+        sc = np.empty((lq, ncols), np.float32)
+        md = np.empty((lq, ncols), np.float32)
+        if threads is None:
+            threads = min(16, os.cpu_count() or 1)
+        self.L.orc_score_matrix(ctypes.byref(self.tables), ctypes.cast(q.ptr, ctypes.c_void_p), qp,
+                                ctypes.cast(db.ptr, ctypes.c_void_p), p0, p1, sc.ctypes.data, md.ctypes.data,
+                                threads)
+        return sc, md
+
+    def ddms_tab(self, tab22, qrec, trec):
+        tab = np.ascontiguousarray(tab22, np.float32)
+        q = np.ascontiguousarray(qrec, np.uint8)
+        t = np.ascontiguousarray(trec, np.uint8)
+        m = np.zeros(3, np.float32)
+        c = np.zeros(3, np.uint32)
+        v = self.L.orc_ddms_tab(tab.ctypes.data, q.ctypes.data, t.ctypes.data, m.ctypes.data, c.ctypes.data)
+        return v, m, c
+
+
+_cache = None
+
+
+def load():
+    global _cache
+    if _cache is None:
+        if not os.path.exists(OLIB):
+            subprocess.run(["make", "-s", "-C", ODIR], check=True)
+        _cache = Oracle(ctypes.CDLL(OLIB))
+    return _cache
+
+
+def load_ref_ddms():
+    """The reference's own CPU DDMS built from its sources (None if absent)."""
+    if not os.path.exists(REFLIB):
+        return None
+    L = ctypes.CDLL(REFLIB)
+    L.ref_ddms_calculate.restype = ctypes.c_float
+    L.ref_ddms_calculate.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    return L
