@@ -63,6 +63,8 @@ def lib():
     L.cth_upload_db.argtypes = [ctypes.c_void_p, P]
     L.cth_score_block.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_void_p, ctypes.c_void_p]
+    L.cth_phase1_block.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                   ctypes.c_void_p, ctypes.c_void_p]
     _lib = L
     return L
 
@@ -152,3 +154,23 @@ class Context:
         md = np.empty((lq, ncols), np.float32)
         _check(lib().cth_score_block(self.h, qidx, p0, p1, sc.ctypes.data, md.ctypes.data), "cth_score_block")
         return sc, md
+
+
+DPRES = np.dtype([("maxsc", np.float32), ("maxxy", np.uint32), ("passed", np.int32), ("score", np.float32),
+                  ("psts", np.int32), ("beg", np.uint32)])
+
+
+def _phase1_block(self, qidx, p0, p1, scorethld, want_btck=False):
+    res = np.zeros(p1 - p0, DPRES)
+    bt = None
+    if want_btck:
+        lq = int(self.qs.lens()[qidx])
+        d, ln = self.db.dists(), self.db.lens()
+        ncols = int(d[p1 - 1] + ln[p1 - 1] - d[p0])
+        bt = np.zeros((lq, ncols), np.uint8)
+    _check(lib().cth_phase1_block(self.h, qidx, p0, p1, scorethld, res.ctypes.data,
+                                  bt.ctypes.data if bt is not None else None), "cth_phase1_block")
+    return res, bt
+
+
+Context.phase1_block = _phase1_block

@@ -13,6 +13,7 @@
 #include <mutex>
 
 #include "cth_ctx.h"
+#include "cth_dp.h"
 
 namespace {
 thread_local char g_err[1024] = "";
@@ -125,5 +126,42 @@ extern "C" int cth_score_block(cth_ctx* c, int qidx, int p0, int p1, float* h_sc
                             lq, hipMemcpyDeviceToHost));
   CTH_HIP_CHECK(hipMemcpy2D(h_mods, sizeof(float) * ncols, c->mods, sizeof(float) * ldsc, sizeof(float) * ncols,
                             lq, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int cth_launch_dp(const cth_dpset* q, int qp, const cth_dpset* db, int p0, int np, int ltmax,
+                             int64_t col0, int64_t ldsc, const float* scores, uint8_t* btck, float scorethld,
+                             cth_dpres* res, hipStream_t st);
+
+// Phase 1 (score matrix, DP, finalisation) of query qidx against template
+// profiles [p0,p1); per-profile results to res (p1-p0 records of 6 words:
+// maxsc, maxxy, passed, score, psts, beg) and, when btck is not NULL, the
+// traceback bytes [Lq][ncols] to host.
+extern "C" int cth_phase1_block(cth_ctx* c, int qidx, int p0, int p1, float scorethld, void* res, uint8_t* h_btck) {
+  if (!c->has_q || !c->has_db) { cth_set_error("phase1_block: queries or database not uploaded"); return -1; }
+  if (qidx < 0 || qidx >= c->q.d.nprof) { cth_set_error("phase1_block: query %d out of range", qidx); return -1; }
+  if (p0 < 0 || p1 > c->db.d.nprof || p0 >= p1) { cth_set_error("phase1_block: bad profile range [%d,%d)", p0, p1); return -1; }
+  CTH_HIP_CHECK(hipSetDevice(c->device));
+  const int lq = c->q.h_len[qidx];
+  const int64_t col0 = c->db.h_dist[p0];
+  const int64_t ncols = c->db.h_dist[p1 - 1] + c->db.h_len[p1 - 1] - col0;
+  if (ncols > (1ll << 30)) { cth_set_error("phase1_block: too many columns"); return -1; }
+  const int64_t ldsc = (ncols + 63) / 64 * 64;
+  const size_t need = sizeof(float) * (size_t)lq * ldsc;
+  int ltmax = 0;
+  for (int p = p0; p < p1; p++) ltmax = c->db.h_len[p] > ltmax ? c->db.h_len[p] : ltmax;
+  if (cth_ctx_reserve(c, (void**)&c->scores, &c->cap_scores, need)) return -1;
+  if (cth_ctx_reserve(c, (void**)&c->mods, &c->cap_mods, need)) return -1;
+  if (cth_ctx_reserve(c, (void**)&c->btck, &c->cap_btck, (size_t)lq * ldsc)) return -1;
+  if (cth_ctx_reserve(c, (void**)&c->dpout, &c->cap_dpout, sizeof(cth_dpres) * (size_t)(p1 - p0))) return -1;
+  if (cth_launch_score(&c->q.d, qidx, &c->db.d, col0, (int)ncols, ldsc, c->dtab, lq, c->scores, c->mods, c->st))
+    return -1;
+  if (cth_launch_dp(&c->q.d, qidx, &c->db.d, p0, p1 - p0, ltmax, col0, ldsc, c->scores, c->btck, scorethld,
+                    (cth_dpres*)c->dpout, c->st))
+    return -1;
+  CTH_HIP_CHECK(hipStreamSynchronize(c->st));
+  CTH_HIP_CHECK(hipMemcpy(res, c->dpout, sizeof(cth_dpres) * (size_t)(p1 - p0), hipMemcpyDeviceToHost));
+  if (h_btck)
+    CTH_HIP_CHECK(hipMemcpy2D(h_btck, ncols, c->btck, ldsc, ncols, lq, hipMemcpyDeviceToHost));
   return 0;
 }

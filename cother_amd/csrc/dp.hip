@@ -1,0 +1,265 @@
+// dp.hip: phase-1 profile-profile DP with traceback, and its finalisation.
+//
+// k_dp computes, for one template profile per wavefront, the five-state
+// local DP of ExecDP_Btck_Unroll32x (cudp/CuBatchDP_init_btck.cu:65-494)
+// over the whole Lq x Lt matrix.  MI355X layout: the 64 lanes of a
+// wavefront own 64 consecutive query rows (a stripe) and sweep the
+// anti-diagonals of the stripe; each step a lane takes its upper
+// neighbour's state (and the template column's transitions) from lane-1 by a
+// DPP wave_shr:1 move, so the wavefront advances one anti-diagonal per step
+// with no LDS round trip.  The bottom row of a stripe is kept in LDS for the
+// next stripe.  Cell values equal the reference's: rows start at column
+// -1-(q mod 32) with zero boundaries, rows q = 0 mod 32 see zeros above at
+// negative columns, rows q = 31 mod 32 see zeros left and upper-left at
+// column 0, out-of-range transitions are log(0) = -32768 and cells
+// outside the valid area score -9 -- the boundaries the reference's 32x32
+// oblique blocks produce.  The maximum is the first maximum in row-major
+// order (per-lane strict max, then a (value, row, column) reduction).
+//
+// k_dpfin restates FinalizeDPPro (cudp/CuBatchDP_final.cu:61-292): pass
+// test against the score threshold, traceback from the maximum, correlation
+// bonus over DIAG scores in 32-score chunks (each chunk reduced in the
+// reference's xor-butterfly order), signed square root, positive-score count
+// and begin coordinates.
+#include <hip/hip_runtime.h>
+
+#include "cth_dev.h"
+#include "cth_dp.h"
+
+namespace {
+
+constexpr float kLog0 = CTH_LOG0;
+
+__device__ __forceinline__ float dpp_shr1(float old_lane0, float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old_lane0),
+                                                               __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, false));
+}
+
+__device__ __forceinline__ float mx(float a, float b) { return a < b ? b : a; }
+
+__device__ __forceinline__ int corner_maxlen(int lq, int lt) {
+  int m = (int)rintf(CTH_CORNERPRC * (float)(lq > lt ? lq : lt));
+  return m > CTH_MAXCORNER ? CTH_MAXCORNER : m;
+}
+
+__device__ __forceinline__ bool in_valid_area(int lq, int lt, int edge, int y, int x) {
+  const bool corners = (lq <= y + edge && x <= edge + y - lq) || (lt - edge + y <= x && y < edge);
+  return !corners && 0 <= y && y < lq && 0 <= x && x < lt;
+}
+
+__global__ __launch_bounds__(64) void k_dp(cth_dpset q, int qp, cth_dpset db, int pbase, int64_t col0, int64_t ldsc,
+                                           const float* __restrict__ scores, uint8_t* __restrict__ btck,
+                                           cth_dpres* __restrict__ res) {
+  extern __shared__ float bot[];  // [5][ltmax]: bottom row of the previous stripe
+  const int lane = threadIdx.x;
+  const int p = pbase + blockIdx.x;
+  const int lq = q.len[qp], lt = db.len[p];
+  const int64_t colp = db.dist[p] - col0;
+  const int64_t qb = q.dist[qp] + qp;      // query transition vector 0
+  const int64_t tb = db.dist[p] + p;       // template transition vector 0
+  const int edge = corner_maxlen(lq, lt);
+  const int ltmax = lt;  // row stride of bot (the launch reserves 5 x the longest profile)
+  float best = 0.0f;
+  uint32_t bestxy = 0;
+  const int nstripes = (lq + 63) >> 6;
+  const int c0 = -1 - (lane & 31);
+  for (int s = 0; s < nstripes; s++) {
+    const int y = (s << 6) + lane;
+    const bool rowok = y < lq;
+    float qt[CTH_NTRN];
+    if (rowok) {
+      qt[CTH_MM] = q.trn[CTH_MM * q.ldtr + qb + y];
+      qt[CTH_MI] = q.trn[CTH_MI * q.ldtr + qb + y + 1];
+      qt[CTH_MD] = q.trn[CTH_MD * q.ldtr + qb + y];
+      qt[CTH_IM] = q.trn[CTH_IM * q.ldtr + qb + y];
+      qt[CTH_II] = q.trn[CTH_II * q.ldtr + qb + y + 1];
+      qt[CTH_DM] = q.trn[CTH_DM * q.ldtr + qb + y];
+      qt[CTH_DD] = q.trn[CTH_DD * q.ldtr + qb + y];
+    } else {
+#pragma unroll
+      for (int i = 0; i < CTH_NTRN; i++) qt[i] = kLog0;
+    }
+    float cMM = 0, cMI = 0, cIM = 0, cDN = 0, cND = 0;     // state of this lane's last column
+    float pMM = 0, pMI = 0, pIM = 0, pDN = 0, pND = 0;     // up-state received last step (= diagonal now)
+    float dt[CTH_NTRN];
+#pragma unroll
+    for (int i = 0; i < CTH_NTRN; i++) dt[i] = kLog0;
+    const int nsteps = lt + 95;
+    for (int t = 0; t < nsteps; t++) {
+      const int c = t - lane - 32;
+      // template transitions: lane 0 loads column t-32, the others take lane-1's
+      const int cu = t - 32;
+      float nt[CTH_NTRN];
+      if (0 <= cu && cu < lt) {
+        nt[CTH_MM] = db.trn[CTH_MM * db.ldtr + tb + cu];
+        nt[CTH_MI] = db.trn[CTH_MI * db.ldtr + tb + cu + 1];
+        nt[CTH_MD] = db.trn[CTH_MD * db.ldtr + tb + cu];
+        nt[CTH_IM] = db.trn[CTH_IM * db.ldtr + tb + cu];
+        nt[CTH_II] = db.trn[CTH_II * db.ldtr + tb + cu + 1];
+        nt[CTH_DM] = db.trn[CTH_DM * db.ldtr + tb + cu];
+        nt[CTH_DD] = db.trn[CTH_DD * db.ldtr + tb + cu];
+      } else {
+#pragma unroll
+        for (int i = 0; i < CTH_NTRN; i++) nt[i] = kLog0;
+      }
+#pragma unroll
+      for (int i = 0; i < CTH_NTRN; i++) dt[i] = dpp_shr1(nt[i], dt[i]);
+      // up state: lane 0 reads the previous stripe's bottom row (zero for the
+      // first stripe and for negative columns), the others take lane-1's
+      float b0 = 0, b1 = 0, b2 = 0, b3 = 0, b4 = 0;
+      if (s > 0 && 0 <= cu && cu < lt) {
+        b0 = bot[0 * ltmax + cu]; b1 = bot[1 * ltmax + cu]; b2 = bot[2 * ltmax + cu];
+        b3 = bot[3 * ltmax + cu]; b4 = bot[4 * ltmax + cu];
+      }
+      float uMM = dpp_shr1(b0, cMM), uMI = dpp_shr1(b1, cMI), uIM = dpp_shr1(b2, cIM);
+      float uDN = dpp_shr1(b3, cDN), uND = dpp_shr1(b4, cND);
+      if ((lane & 31) == 0 && c < 0) { uMM = uMI = uIM = uDN = uND = 0.0f; }
+      if ((lane & 31) == 31 && c == 0) {  // thread 0 of the block at x = 0: zero left/diagonal
+        cMM = cMI = cIM = cDN = cND = 0.0f;
+        pMM = pMI = pIM = pDN = pND = 0.0f;
+      }
+      if (rowok && c0 <= c && c < lt) {
+        const float S = in_valid_area(lq, lt, edge, y, c) ? scores[(int64_t)y * ldsc + colp + c] : CTH_OUTSCORE;
+        float v1, v2, m;
+        int bk;
+        v1 = pMI + qt[CTH_MM] + dt[CTH_IM];
+        v2 = pIM + qt[CTH_IM] + dt[CTH_MM];
+        v1 = mx(v1, v2);
+        v2 = pDN + qt[CTH_DM] + dt[CTH_MM];
+        v1 = mx(v1, v2);
+        v2 = pND + qt[CTH_MM] + dt[CTH_DM];
+        v1 = mx(v1, v2);
+        v2 = pMM + qt[CTH_MM] + dt[CTH_MM];
+        v1 = mx(v1, v2);
+        const float nMM = mx(v1 + S, 0.0f);
+        m = nMM;
+        bk = nMM != 0.0f ? CTH_BT_DIAG : CTH_BT_STOP;
+        v1 = uMM + qt[CTH_MM] + dt[CTH_MI];
+        v2 = uMI + qt[CTH_MM] + dt[CTH_II];
+        const float nMI = mx(v1, v2);
+        if (m < nMI) { m = nMI; bk = CTH_BT_UP; }
+        v1 = cMM + qt[CTH_MI] + dt[CTH_MM];
+        v2 = cIM + qt[CTH_II] + dt[CTH_MM];
+        const float nIM = mx(v1, v2);
+        if (m < nIM) { m = nIM; bk = CTH_BT_LEFT; }
+        v1 = uMM + qt[CTH_MD];
+        v2 = uDN + qt[CTH_DD];
+        const float nDN = mx(v1, v2);
+        if (m < nDN) { m = nDN; bk = CTH_BT_UP; }
+        v1 = cMM + dt[CTH_MD];
+        v2 = cND + dt[CTH_DD];
+        const float nND = mx(v1, v2);
+        if (m < nND) { m = nND; bk = CTH_BT_LEFT; }
+        cMM = nMM; cMI = nMI; cIM = nIM; cDN = nDN; cND = nND;
+        if (0 <= c) {
+          if (best < nMM) { best = nMM; bestxy = ((uint32_t)c << 16) | (uint32_t)y; }
+          btck[(int64_t)y * ldsc + colp + c] = (uint8_t)bk;
+        }
+      }
+      pMM = uMM; pMI = uMI; pIM = uIM; pDN = uDN; pND = uND;
+      if (lane == 63 && 0 <= c && c < lt) {
+        bot[0 * ltmax + c] = cMM; bot[1 * ltmax + c] = cMI; bot[2 * ltmax + c] = cIM;
+        bot[3 * ltmax + c] = cDN; bot[4 * ltmax + c] = cND;
+      }
+    }
+    __syncthreads();
+  }
+  // first maximum in row-major order across the wavefront
+  for (int off = 32; off >= 1; off >>= 1) {
+    const float ob = __shfl_xor(best, off);
+    const uint32_t oxy = __shfl_xor(bestxy, off);
+    const uint32_t my = bestxy & 0xffff, oy = oxy & 0xffff;
+    const bool take = ob > best || (ob == best && (oy < my || (oy == my && (oxy >> 16) < (bestxy >> 16))));
+    if (take) { best = ob; bestxy = oxy; }
+  }
+  if (lane == 0) {
+    res[blockIdx.x].maxsc = best;
+    res[blockIdx.x].maxxy = bestxy;
+  }
+}
+
+// lane 0 of each wavefront finalises one profile
+__global__ __launch_bounds__(64) void k_dpfin(cth_dpset db, int pbase, int64_t col0, int64_t ldsc,
+                                              const float* __restrict__ scores, const uint8_t* __restrict__ btck,
+                                              float scorethld, cth_dpres* __restrict__ res) {
+  __shared__ float cache[36];
+  __shared__ float tree[32];
+  if (threadIdx.x != 0) return;
+  const int p = pbase + blockIdx.x;
+  const int64_t colp = db.dist[p] - col0;
+  cth_dpres r = res[blockIdx.x];
+  r.passed = 0; r.score = 0.0f; r.psts = 0; r.beg = 0;
+  if (r.maxsc <= scorethld) { res[blockIdx.x] = r; return; }
+  int x = (int)(r.maxxy >> 16), y = (int)(r.maxxy & 0xffff);
+  int psts = 0;
+  for (int k = 0; k < 36; k++) cache[k] = 0.0f;
+  float sum = 0.0f;
+  int bt = CTH_BT_DIAG;
+  while (bt != CTH_BT_STOP) {
+    for (int k = 0; k < 4; k++) cache[k] = cache[32 + k];
+    int ndx;
+    for (ndx = 4; ndx < 36;) {
+      if (x < 0 || y < 0) { bt = CTH_BT_STOP; break; }
+      const int64_t a = (int64_t)y * ldsc + colp + x;
+      bt = btck[a];
+      if (bt == CTH_BT_STOP) break;
+      if (bt == CTH_BT_UP) { y--; continue; }
+      if (bt == CTH_BT_LEFT) { x--; continue; }
+      const float v = scores[a];
+      cache[ndx] = v;
+      if (v > 0.0f) psts++;
+      x--; y--;
+      ndx++;
+    }
+    for (int l = 0; l < 32; l++)
+      tree[l] = (ndx <= l + 4) ? 0.0f
+                               : cache[l + 4] * (((cache[l] + cache[l + 1]) + cache[l + 2]) + cache[l + 3]);
+    // xor butterfly (offsets 16,8,4,2,1); lane 0's value
+    for (int off = 16; off >= 1; off >>= 1) {
+      float nt[32];
+      for (int l = 0; l < 32; l++) nt[l] = tree[l] + tree[l ^ off];
+      for (int l = 0; l < 32; l++) tree[l] = nt[l];
+    }
+    sum += tree[0];
+  }
+  sum *= 0.5f;
+  sum = sum < 0.0f ? -sqrtf(-sum) : sqrtf(sum);
+  const float score = r.maxsc + sum;
+  r.psts = psts;
+  r.beg = ((uint32_t)(x + 1) << 16) | (uint32_t)(y + 1);
+  r.score = score;
+  r.passed = scorethld < score ? 1 : 0;
+  res[blockIdx.x] = r;
+}
+
+__global__ void k_selftest_dpp(int* out) {
+  const float v = (float)threadIdx.x;
+  out[threadIdx.x] = (int)dpp_shr1(-1.0f, v);
+}
+
+}  // namespace
+
+// Lane-layout check of the DPP wave_shr:1 move the DP kernel relies on:
+// out[l] = l-1 for l >= 1, out[0] = -1.
+extern "C" int cth_selftest_dpp(int* host_out64) {
+  int* d = nullptr;
+  CTH_HIP_CHECK(hipMalloc(&d, 64 * sizeof(int)));
+  hipLaunchKernelGGL(k_selftest_dpp, dim3(1), dim3(64), 0, 0, d);
+  CTH_HIP_CHECK(hipGetLastError());
+  CTH_HIP_CHECK(hipMemcpy(host_out64, d, 64 * sizeof(int), hipMemcpyDeviceToHost));
+  CTH_HIP_CHECK(hipFree(d));
+  return 0;
+}
+
+extern "C" int cth_launch_dp(const cth_dpset* q, int qp, const cth_dpset* db, int p0, int np, int ltmax,
+                             int64_t col0, int64_t ldsc, const float* scores, uint8_t* btck, float scorethld,
+                             cth_dpres* res, hipStream_t st) {
+  if (np <= 0) return 0;
+  const size_t lds = sizeof(float) * 5 * (size_t)(ltmax > 0 ? ltmax : 1);
+  if (lds > 160 * 1024) { cth_set_error("dp: profile length %d exceeds the LDS budget", ltmax); return -1; }
+  hipLaunchKernelGGL(k_dp, dim3(np), dim3(64), lds, st, *q, qp, *db, p0, col0, ldsc, scores, btck, res);
+  CTH_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_dpfin, dim3(np), dim3(64), 0, st, *db, p0, col0, ldsc, scores, btck, scorethld, res);
+  CTH_HIP_CHECK(hipGetLastError());
+  return 0;
+}

@@ -75,6 +75,18 @@ int cth_upload_db(cth_ctx* c, const cth_pset* db);
  * this entry exposes its outputs (scores, modscores) for inspection. */
 int cth_score_block(cth_ctx* c, int qidx, int p0, int p1, float* scores, float* modscores);
 
+/* Phase 1 of query qidx against template profiles [p0,p1): score matrix,
+ * DP with traceback (CuBatchDP::PerformCompleteDynProgDevice,
+ * src/libmycu/cudp/CuBatchDP.cu:65-118) and its finalisation (FinalizeDPPro,
+ * src/libmycu/cudp/CuBatchDP_final.cu:61-292).  res receives p1-p0 records
+ * of six 32-bit words {maxsc(f32), maxxy(u32), passed(i32), score(f32),
+ * psts(i32), beg(u32)}; btck (optional) the traceback bytes [Lq][ncols]. */
+int cth_phase1_block(cth_ctx* c, int qidx, int p0, int p1, float scorethld, void* res, uint8_t* btck);
+
+/* Lane-layout self-check of the cross-lane move the DP uses (no reference
+ * counterpart): out[l] = l-1 for l >= 1, out[0] = -1. */
+int cth_selftest_dpp(int* out64);
+
 #ifdef __cplusplus
 }
 #endif

@@ -78,6 +78,13 @@ float orc_ddms_tab(const float* tab22, const uint8_t* qrec, const uint8_t* trec,
 void orc_score_matrix(const orc_tables* t, const orc_pset* q, int qp, const orc_pset* db, int p0, int p1,
                       float* scores, float* modscores, int nthreads);
 
+/* phase-1 DP + finalisation (orc_dp.c) */
+void orc_dp_phase1(const orc_pset* q, int qp, const orc_pset* db, int p0, int p1, const float* sc, uint8_t* bt,
+                   float scorethld, float* maxsc, uint32_t* maxxy, int* passed, float* score, int* psts,
+                   uint32_t* beg, int nthreads);
+void orc_score_threshold(float eval, float lambda, float K, uint64_t nqyposs, uint64_t dbsize, float* scorethld,
+                         float* logevthld);
+
 #ifdef __cplusplus
 }
 #endif

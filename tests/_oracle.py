@@ -48,6 +48,11 @@ class Oracle:
         L.orc_ddms_tab.argtypes = [vp, vp, vp, vp, vp]
         L.orc_score_matrix.argtypes = [ctypes.POINTER(OrcTables), vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int,
                                        vp, vp, ctypes.c_int]
+        L.orc_dp_phase1.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_float,
+                                    vp, vp, vp, vp, vp, vp, ctypes.c_int]
+        L.orc_score_threshold.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_uint64,
+                                          ctypes.c_uint64, ctypes.POINTER(ctypes.c_float),
+                                          ctypes.POINTER(ctypes.c_float)]
         self.tables = OrcTables()
         from cother_amd import ENVDIR
         rc = L.orc_tables_load(ENVDIR.encode(), ctypes.byref(self.tables))
@@ -69,6 +74,30 @@ class Oracle:
                                 ctypes.cast(db.ptr, ctypes.c_void_p), p0, p1, sc.ctypes.data, md.ctypes.data,
                                 threads)
         return sc, md
+
+    def phase1(self, q, qp, db, p0, p1, scores, scorethld, threads=None):
+        """DP + finalisation; returns (records as DPRES array, btck)."""
+        from cother_amd import DPRES
+        n = p1 - p0
+        lq, ncols = scores.shape
+        bt = np.zeros((lq, ncols), np.uint8)
+        maxsc = np.zeros(n, np.float32); maxxy = np.zeros(n, np.uint32); passed = np.zeros(n, np.int32)
+        score = np.zeros(n, np.float32); psts = np.zeros(n, np.int32); beg = np.zeros(n, np.uint32)
+        if threads is None:
+            threads = min(16, os.cpu_count() or 1)
+        sc = np.ascontiguousarray(scores, np.float32)
+        self.L.orc_dp_phase1(ctypes.cast(q.ptr, ctypes.c_void_p), qp, ctypes.cast(db.ptr, ctypes.c_void_p), p0, p1,
+                             sc.ctypes.data, bt.ctypes.data, scorethld, maxsc.ctypes.data, maxxy.ctypes.data,
+                             passed.ctypes.data, score.ctypes.data, psts.ctypes.data, beg.ctypes.data, threads)
+        res = np.zeros(n, DPRES)
+        res["maxsc"], res["maxxy"], res["passed"] = maxsc, maxxy, passed
+        res["score"], res["psts"], res["beg"] = score, psts, beg
+        return res, bt
+
+    def score_threshold(self, eval_, nqyposs, dbsize, lam=0.561132, K=0.001488):
+        s = ctypes.c_float(); le = ctypes.c_float()
+        self.L.orc_score_threshold(eval_, lam, K, nqyposs, dbsize, ctypes.byref(s), ctypes.byref(le))
+        return s.value, le.value
 
     def ddms_tab(self, tab22, qrec, trec):
         tab = np.ascontiguousarray(tab22, np.float32)
