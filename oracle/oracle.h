@@ -85,6 +85,18 @@ void orc_dp_phase1(const orc_pset* q, int qp, const orc_pset* db, int p0, int p1
 void orc_score_threshold(float eval, float lambda, float K, uint64_t nqyposs, uint64_t dbsize, float* scorethld,
                          float* logevthld);
 
+uint64_t orc_length_adjustment(float lambda, float K, float alpha, float beta, uint64_t m, uint64_t n, uint64_t N,
+                               uint64_t* dl);
+
+/* statistics (orc_stats.c), double precision */
+void orc_histogram(const float* sc, int64_t ldsc, int64_t col, int lq, int lt, const char* qaa, const char* taa,
+                   double counts[641]);
+void orc_stat_params(const double counts[641], double out[6]);
+double orc_lbetainc(double a, double b, double x);
+double orc_lgammaincq(double a, double x);
+void orc_significance(int ssemodel, int lq, int lt, double qeno, double teno, double searchspace, double score,
+                      double psts, double lambda, double K, double out[4]);
+
 #ifdef __cplusplus
 }
 #endif

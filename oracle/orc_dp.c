@@ -240,3 +240,46 @@ void orc_score_threshold(float eval, float lambda, float K, uint64_t nqyposs, ui
   if (s < 0.0f) s = 0.0f;
   *scorethld = s;
 }
+
+/* AbstractScoreMatrix::ComputeLengthAdjustment (libpro/srcsco/AbstractScoreMatrix.cpp:303-395):
+ * Gertz/Altschul length correction; returns the search space, delta length in *dl */
+uint64_t orc_length_adjustment(float lambda, float K, float alpha, float beta, uint64_t m, uint64_t n, uint64_t N,
+                               uint64_t* dl) {
+  *dl = 0;
+  if (lambda <= 0.0f || K <= 0.0f || N < 1) return m * n;
+  const int maxit = 20;
+  const float logK = logf(K);
+  const float alratio = alpha / lambda;
+  const float max_mn = (float)((m > n) ? m : n);
+  float length = 0.0f, min_length = 0.0f, max_length = 0.0f, nxt_length = 0.0f;
+  float sspace = 0.0f, space2 = 0.0f;
+  int converged = 0;
+  const float a = (float)N;
+  const float b = (float)(n + m * N);
+  const float c = (float)(m * n) - max_mn / K;
+  if (c < 0.0f) return m * n;
+  max_length = 2.0f * c / (b + sqrtf(b * b - 4.0f * a * c));
+  for (int j = 0; j < maxit; j++) {
+    length = nxt_length;
+    sspace = (m - length) * (n - N * length);
+    nxt_length = alratio * (logK + logf(sspace)) + beta;
+    if (length <= nxt_length) {
+      min_length = length;
+      if (nxt_length - length <= 1.0f) { converged = 1; break; }
+      if (min_length == max_length) break;
+    } else
+      max_length = length;
+    if (nxt_length < min_length || max_length < nxt_length)
+      nxt_length = (!j) ? max_length : (min_length + max_length) * 0.5f;
+  }
+  if (converged) {
+    length = ceilf(min_length);
+    if (length <= max_length) {
+      space2 = (m - length) * (n - N * length);
+      nxt_length = alratio * (logK + logf(space2)) + beta;
+      if (length <= nxt_length) { min_length = length; sspace = space2; }
+    }
+  }
+  *dl = (uint64_t)min_length;
+  return (uint64_t)sspace;
+}
