@@ -65,6 +65,11 @@ def lib():
                                   ctypes.c_void_p, ctypes.c_void_p]
     L.cth_phase1_block.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                                    ctypes.c_void_p, ctypes.c_void_p]
+    L.cth_selftest_dpp.argtypes = [ctypes.c_void_p]
+    L.cth_device_count.argtypes = []
+    L.cth_phase2_block.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                   ctypes.c_double, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                   ctypes.POINTER(ctypes.c_int), ctypes.c_void_p]
     _lib = L
     return L
 
@@ -174,3 +179,18 @@ def _phase1_block(self, qidx, p0, p1, scorethld, want_btck=False):
 
 
 Context.phase1_block = _phase1_block
+
+
+def _phase2_block(self, qidx, p0, p1, scorethld, searchspace, ssemodel=2):
+    n = p1 - p0
+    res = np.zeros(n, DPRES)
+    plist = np.zeros(n, np.int32)
+    stats = np.zeros((n, 10), np.float64)
+    npass = ctypes.c_int()
+    _check(lib().cth_phase2_block(self.h, qidx, p0, p1, scorethld, searchspace, ssemodel, res.ctypes.data,
+                                  plist.ctypes.data, ctypes.byref(npass), stats.ctypes.data), "cth_phase2_block")
+    k = npass.value
+    return res, plist[:k], stats[:k]
+
+
+Context.phase2_block = _phase2_block

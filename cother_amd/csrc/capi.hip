@@ -14,6 +14,8 @@
 
 #include "cth_ctx.h"
 #include "cth_dp.h"
+#include "cth_stats.h"
+#include <vector>
 
 namespace {
 thread_local char g_err[1024] = "";
@@ -163,5 +165,48 @@ extern "C" int cth_phase1_block(cth_ctx* c, int qidx, int p0, int p1, float scor
   CTH_HIP_CHECK(hipMemcpy(res, c->dpout, sizeof(cth_dpres) * (size_t)(p1 - p0), hipMemcpyDeviceToHost));
   if (h_btck)
     CTH_HIP_CHECK(hipMemcpy2D(h_btck, ncols, c->btck, ldsc, ncols, lq, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int cth_launch_stats(const cth_dpset* q, int qp, const cth_dpset* db, int p0, int64_t col0, int64_t ldsc,
+                                const float* scores, const int* plist_abs, const int* plist_rel,
+                                const cth_dpres* dpres, int np2, double searchspace, int ssemodel, int* hist,
+                                double* out, hipStream_t st);
+
+// Phase 1 and the statistics of the passed profiles for query qidx against
+// template profiles [p0,p1).  res: p1-p0 phase-1 records (as cth_phase1_block);
+// plist: indices (relative to p0) of the passed profiles, *npass their number;
+// stats: npass x 10 doubles {lambda_est, K_est, logE, logEpA, min, max, E, H,
+// lambda, K}.  plist and stats must hold p1-p0 entries.
+extern "C" int cth_phase2_block(cth_ctx* c, int qidx, int p0, int p1, float scorethld, double searchspace,
+                                int ssemodel, void* res, int* plist, int* npass, double* stats) {
+  if (cth_phase1_block(c, qidx, p0, p1, scorethld, res, nullptr)) return -1;
+  const int n = p1 - p0;
+  const cth_dpres* r = (const cth_dpres*)res;
+  std::vector<int> rel, abs_;
+  for (int k = 0; k < n; k++)
+    if (r[k].passed) { rel.push_back(k); abs_.push_back(p0 + k); }
+  *npass = (int)rel.size();
+  for (size_t i = 0; i < rel.size(); i++) plist[i] = rel[i];
+  if (rel.empty()) return 0;
+  const int np2 = (int)rel.size();
+  const int lq = c->q.h_len[qidx];
+  const int64_t col0 = c->db.h_dist[p0];
+  const int64_t ncols = c->db.h_dist[p1 - 1] + c->db.h_len[p1 - 1] - col0;
+  const int64_t ldsc = (ncols + 63) / 64 * 64;
+  (void)lq;
+  const size_t need = sizeof(int) * 2 * np2 + sizeof(int) * 641 * (size_t)np2 + sizeof(double) * 10 * (size_t)np2 + 256;
+  if (cth_ctx_reserve(c, &c->work, &c->cap_work, need)) return -1;
+  int* d_abs = (int*)c->work;
+  int* d_rel = d_abs + np2;
+  int* d_hist = d_rel + np2;
+  double* d_out = (double*)(((uintptr_t)(d_hist + 641 * (size_t)np2) + 15) & ~(uintptr_t)15);
+  CTH_HIP_CHECK(hipMemcpyAsync(d_abs, abs_.data(), sizeof(int) * np2, hipMemcpyHostToDevice, c->st));
+  CTH_HIP_CHECK(hipMemcpyAsync(d_rel, rel.data(), sizeof(int) * np2, hipMemcpyHostToDevice, c->st));
+  if (cth_launch_stats(&c->q.d, qidx, &c->db.d, p0, col0, ldsc, c->scores, d_abs, d_rel, (const cth_dpres*)c->dpout,
+                       np2, searchspace, ssemodel, d_hist, d_out, c->st))
+    return -1;
+  CTH_HIP_CHECK(hipStreamSynchronize(c->st));
+  CTH_HIP_CHECK(hipMemcpy(stats, d_out, sizeof(double) * 10 * np2, hipMemcpyDeviceToHost));
   return 0;
 }

@@ -83,6 +83,14 @@ int cth_score_block(cth_ctx* c, int qidx, int p0, int p1, float* scores, float* 
  * psts(i32), beg(u32)}; btck (optional) the traceback bytes [Lq][ncols]. */
 int cth_phase1_block(cth_ctx* c, int qidx, int p0, int p1, float scorethld, void* res, uint8_t* btck);
 
+/* Phase 1 plus the alignment statistics of the passed profiles
+ * (CuBatchSS::CalculateAlnStatisticsDevice, src/libmycu/cuss/CuBatchSS.cu:
+ * 63-266): res as cth_phase1_block; plist receives the indices (relative to
+ * p0) of the *npass passed profiles; stats npass x 10 doubles {lambda_est,
+ * K_est, logE, logEpA, min, max, E, H, lambda, K}. */
+int cth_phase2_block(cth_ctx* c, int qidx, int p0, int p1, float scorethld, double searchspace, int ssemodel,
+                     void* res, int* plist, int* npass, double* stats);
+
 /* Lane-layout self-check of the cross-lane move the DP uses (no reference
  * counterpart): out[l] = l-1 for l >= 1, out[0] = -1. */
 int cth_selftest_dpp(int* out64);

@@ -28,6 +28,14 @@ def test_library_exports_declared_symbols():
     assert not missing, missing
 
 
+def test_bindings_declare_argument_types():
+    """every declared entry point is bound with explicit argument types (no
+    implicit int conversion of pointers)"""
+    L = cother_amd.lib()
+    unbound = [n for n in declared_functions() if getattr(L, n).argtypes is None]
+    assert not unbound, unbound
+
+
 def test_synth_deterministic():
     a = cother_amd.synth(5, 0, lmin=40, lmax=90, dist_mode=1, seed=3, threads=1)
     b = cother_amd.synth(5, 0, lmin=40, lmax=90, dist_mode=1, seed=3, threads=4)
