@@ -57,6 +57,7 @@ def lib():
     L.cth_pset_alloc.restype = P
     L.cth_pset_alloc.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int32)]
     L.cth_last_error.restype = ctypes.c_char_p
+    L.cth_last_error.argtypes = []
     L.cth_ctx_create.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]
     L.cth_ctx_destroy.argtypes = [ctypes.c_void_p]
     L.cth_upload_queries.argtypes = [ctypes.c_void_p, P]
