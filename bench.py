@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Search-throughput benchmark of the MI355X COTHER hot path.
+
+Workload: 64 query profiles (lengths lognormal, median 250, sigma 0.35,
+clipped 40-1000, predicted distance maps) against an HBM-resident database of
+60,000 template profiles (lengths lognormal, median 220, sigma 0.35, clipped
+30-1500, C-alpha chain distances; about 14.0M positions), all seeded
+synthetic profiles in the reference's value conventions.
+
+A step is one query searched against the whole resident database: score
+matrix (log-odds + SS + CVS + DDMS), phase-1 DP with traceback and
+finalisation, statistics and E-values of the passed profiles, E-value filter
+and hit ordering.  Step s uses query s mod 64.  Cells = Lq x database
+positions; value = cells of the timed steps / wall time of the timed steps.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+METRIC = "GCUPS (score+DP+MAP) over resident DB; p50/p99 query latency vs plain FP32"
+
+# MI355X published peaks
+PEAK_VALU_FP32_TFLOPS = 157.3   # FP32 vector
+PEAK_HBM_GBS = 8000.0
+
+# algorithmic FP32 operations per score cell outside DDMS: log-odds over 20
+# amino acids (5 ops each + log), SS (9 products, 9 table terms, H term),
+# CVS (19 multiply-adds, norms, determinant, log, piecewise-linear map)
+OPS_CELL_BASE = 216
+# algorithmic FP32 operations per DDMS segment-cell update: add, clamp at 0,
+# compare with the step maximum, plus one per update for the interpolated cells
+OPS_DDMS_UPDATE = 4
+DSTSEGM = 6
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def ddms_updates_table(nmax=128):
+    """U[n1][n2]: segment-cell updates of one DDMS evaluation for list lengths
+    n1 (query) and n2 (template): upper diagonals j = n2>>2 .. 0 step 2 with
+    ceil(min(n1, n2-j)/4) steps, lower diagonals i = n1>>2 .. 1 step 2 with
+    ceil(min(n2, n1-i)/4) steps, 4 updates per step; 0 if either list is
+    shorter than 6 (CalcDDMS2ScoreSMEMh2, cusco/CuBatchScoreMatrix_ddms2s_h2.cuh:435-500)."""
+    U = np.zeros((nmax, nmax), np.int64)
+    n = np.arange(nmax)
+    for n1 in range(DSTSEGM, nmax):
+        for n2 in range(DSTSEGM, nmax):
+            s = 0
+            for j in range(n2 >> 2, -1, -2):
+                s += (min(n1, n2 - j) + 3) // 4
+            for i in range(n1 >> 2, 0, -2):
+                s += (min(n2, n1 - i) + 3) // 4
+            U[n1, n2] = 4 * s
+    return U
+
+
+def algorithmic_ops(qlens_hist, db_hist, lq, npos, U):
+    """Algorithmic FP32 ops of one query's score matrix against the database."""
+    ddms = float(qlens_hist @ U @ db_hist)
+    return OPS_DDMS_UPDATE * ddms + OPS_CELL_BASE * float(lq) * float(npos)
+
+
+def read_traffic():
+    """HBM bytes per score cell of the score kernel from the committed rocprofv3
+    --pmc summary (line 'bytes_per_cell: X'), or None."""
+    p = os.path.join(ROOT, "profiles", "k_score_c1_pmc.txt")
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("bytes_per_cell:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def cpu_baseline(q, db, target_s, threads):
+    """The CPU oracle (the C restatement of the same pipeline, OpenMP over
+    templates) on a bounded sample: query 0 against a contiguous block of
+    templates from the middle of the length-sorted database, grown until the
+    run takes about target_s seconds."""
+    sys.path.insert(0, ROOT)
+    from tests import _oracle
+    orc = _oracle.load()
+    lens = db.lens()
+    lq = int(q.lens()[0])
+    p0 = db.nprof // 2
+    nblk = 8
+    while True:
+        p1 = min(db.nprof, p0 + nblk)
+        t0 = time.perf_counter()
+        orc.search_query(q, 0, db, p0, p1, 10.0, 2, db.npos, db.nprof, threads)
+        dt = time.perf_counter() - t0
+        cells = float(lq) * float(lens[p0:p1].sum())
+        if dt >= target_s or p1 == db.nprof:
+            break
+        nblk = int(nblk * min(8.0, max(1.5, 1.2 * target_s / max(dt, 1e-3))))
+    return {"value": cells / dt / 1e9, "unit": "GCUPS", "cores": threads, "kind": "port",
+            "sample": f"query 0 (Lq={lq}) vs templates [{p0},{p1}) ({int(lens[p0:p1].sum())} positions), "
+                      f"{cells:.3g} cells in {dt:.1f} s, full pipeline"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=16)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--queries", type=int, default=64)
+    ap.add_argument("--templates", type=int, default=60000)
+    ap.add_argument("--eval", type=float, default=10.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample size in seconds (0: skip)")
+    ap.add_argument("--seed", type=int, default=20260)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the search path shards over GPUs by database chunk "
+              "(one process per GPU), and only the single-GPU figure is produced here; run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import cother_amd
+    threads = min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    qs = cother_amd.synth(args.queries, 1, 250.0, 0.35, 40, 1000, dist_mode=0, sort_desc=0, seed=args.seed,
+                          threads=threads)
+    db = cother_amd.synth(args.templates, 1, 220.0, 0.35, 30, 1500, dist_mode=1, sort_desc=1,
+                          seed=args.seed + 1, threads=threads)
+    log(f"workload: {qs.nprof} queries ({qs.npos} positions), {db.nprof} templates ({db.npos} positions); "
+        f"generated in {time.perf_counter() - t0:.1f} s")
+    ctx = cother_amd.Context(0)
+    ctx.upload_queries(qs)
+    ctx.upload_db(db)
+    log("uploaded")
+
+    U = ddms_updates_table()
+    db_hist = np.bincount(db.field("dd")[:, 0], minlength=128)[:128].astype(np.float64)
+    qdd = qs.field("dd")[:, 0]
+    qd, ql = qs.dists(), qs.lens()
+
+    for s in range(args.warmup):
+        qi = s % qs.nprof
+        ctx.search(qi, qi + 1, eval=args.eval)
+        log(f"warmup step {s} (query {qi}, Lq={ql[qi]}) done")
+
+    lat, cells, ops, ms_score, nlaunch, ms_dp, ms_st, npass = [], 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0
+    t_all = time.perf_counter()
+    for s in range(args.steps):
+        qi = (args.warmup + s) % qs.nprof
+        t1 = time.perf_counter()
+        hits, nh, st = ctx.search(qi, qi + 1, eval=args.eval, timing=True)
+        lat.append((time.perf_counter() - t1) * 1e3)
+        cells += st.cells
+        ms_score += st.ms_k_score
+        ms_dp += st.ms_k_dp
+        ms_st += st.ms_k_stats
+        nlaunch += st.nchunks
+        npass += st.npassed
+        qh = np.bincount(qdd[qd[qi]:qd[qi] + ql[qi]], minlength=128)[:128].astype(np.float64)
+        ops += algorithmic_ops(qh, db_hist, ql[qi], db.npos, U)
+        log(f"step {s}: query {qi} Lq={ql[qi]} {lat[-1]:.0f} ms, {int(nh[0])} hits, {int(st.npassed)} passed, "
+            f"score {st.ms_k_score:.0f} ms dp {st.ms_k_dp:.0f} ms stats {st.ms_k_stats:.0f} ms")
+    wall = time.perf_counter() - t_all
+    gcups = cells / wall / 1e9
+
+    achieved = ops / (ms_score * 1e-3) / 1e12 if ms_score > 0 else 0.0
+    bpc = read_traffic()
+    cells_per_launch = cells / max(nlaunch, 1.0)
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_VALU_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_VALU_FP32_TFLOPS, 4),
+                "traffic": (bpc * cells_per_launch) if bpc is not None else None,
+                "kernel": "k_score", "avg_launch_ms": round(ms_score / max(nlaunch, 1.0), 3),
+                "ops_per_launch": ops / max(nlaunch, 1.0)}
+    cpu = None
+    if args.cpu_seconds > 0:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(qs, db, args.cpu_seconds, threads)
+    lat_a = np.array(lat)
+    out = {
+        "metric": METRIC, "value": round(gcups, 4), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "64 queries (Lq lognormal median 250, 40-1000) vs resident DB of 60000 templates "
+                               "(lognormal median 220, 30-1500); one query per step",
+                   "queries": qs.nprof, "templates": db.nprof, "db_positions": int(db.npos),
+                   "query_positions_timed": int(sum(ql[(args.warmup + s) % qs.nprof] for s in range(args.steps))),
+                   "eval": args.eval, "map": False},
+        "p50_ms": round(float(np.percentile(lat_a, 50)), 2), "p99_ms": round(float(np.percentile(lat_a, 99)), 2),
+        "stage_ms": {"score": round(ms_score, 1), "dp": round(ms_dp, 1), "stats": round(ms_st, 1)},
+        "passed_per_query": round(npass / args.steps, 1),
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

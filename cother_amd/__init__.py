@@ -38,6 +38,32 @@ class CthPset(ctypes.Structure):
     ]
 
 
+class SearchOpts(ctypes.Structure):
+    """Mirror of cth_search_opts (include/cother_hip.h)."""
+    _fields_ = [("eval", ctypes.c_float), ("ssemodel", ctypes.c_int), ("do_map", ctypes.c_int),
+                ("timing", ctypes.c_int), ("mem_budget", ctypes.c_uint64)]
+
+
+class SearchStats(ctypes.Structure):
+    """Mirror of cth_search_stats (include/cother_hip.h)."""
+    _fields_ = [(n, ctypes.c_double) for n in ("cells", "ms_phase1", "ms_phase2", "npassed", "nchunks",
+                                               "ms_k_score", "ms_k_dp", "ms_k_stats")]
+
+    def asdict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+# Mirror of cth_hit (include/cother_hip.h)
+HIT = np.dtype([("query", np.int32), ("profile", np.int32), ("score", np.float32), ("logE", np.float32),
+                ("logEpA", np.float32), ("lambda_est", np.float32), ("K_est", np.float32), ("minsc", np.float32),
+                ("maxsc", np.float32), ("expected", np.float32), ("H", np.float32), ("lambda_", np.float32),
+                ("K", np.float32), ("psts", np.int32), ("beg", np.uint32), ("end", np.uint32),
+                ("mapbeg", np.uint32), ("mapend", np.uint32), ("idnts", np.int32), ("ngaps", np.int32),
+                ("alnlen", np.int32), ("pstvs_map", np.int32)])
+
+DPRES = np.dtype([("maxsc", np.float32), ("maxxy", np.uint32), ("passed", np.int32), ("score", np.float32),
+                  ("psts", np.int32), ("beg", np.uint32)])
+
 _lib = None
 
 
@@ -71,6 +97,8 @@ def lib():
     L.cth_phase2_block.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                                    ctypes.c_double, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                    ctypes.POINTER(ctypes.c_int), ctypes.c_void_p]
+    L.cth_search.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(SearchOpts),
+                             ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(SearchStats)]
     _lib = L
     return L
 
@@ -161,37 +189,45 @@ class Context:
         _check(lib().cth_score_block(self.h, qidx, p0, p1, sc.ctypes.data, md.ctypes.data), "cth_score_block")
         return sc, md
 
+    def phase1_block(self, qidx, p0, p1, scorethld, want_btck=False):
+        """Phase-1 records (DPRES) of query qidx vs profiles [p0,p1), optionally the traceback bytes."""
+        res = np.zeros(p1 - p0, DPRES)
+        bt = None
+        if want_btck:
+            lq = int(self.qs.lens()[qidx])
+            d, ln = self.db.dists(), self.db.lens()
+            ncols = int(d[p1 - 1] + ln[p1 - 1] - d[p0])
+            bt = np.zeros((lq, ncols), np.uint8)
+        _check(lib().cth_phase1_block(self.h, qidx, p0, p1, scorethld, res.ctypes.data,
+                                      bt.ctypes.data if bt is not None else None), "cth_phase1_block")
+        return res, bt
 
-DPRES = np.dtype([("maxsc", np.float32), ("maxxy", np.uint32), ("passed", np.int32), ("score", np.float32),
-                  ("psts", np.int32), ("beg", np.uint32)])
+    def phase2_block(self, qidx, p0, p1, scorethld, searchspace, ssemodel=2):
+        """Phase 1 plus statistics: (records, passed indices relative to p0, stats [npass,10])."""
+        n = p1 - p0
+        res = np.zeros(n, DPRES)
+        plist = np.zeros(n, np.int32)
+        stats = np.zeros((n, 10), np.float64)
+        npass = ctypes.c_int()
+        _check(lib().cth_phase2_block(self.h, qidx, p0, p1, scorethld, searchspace, ssemodel, res.ctypes.data,
+                                      plist.ctypes.data, ctypes.byref(npass), stats.ctypes.data),
+               "cth_phase2_block")
+        k = npass.value
+        return res, plist[:k], stats[:k]
 
+    def search(self, q0=0, q1=None, eval=10.0, ssemodel=2, do_map=False, timing=False, mem_budget=0,
+               maxhits=None):
+        """Search queries [q0,q1) against the resident database.
 
-def _phase1_block(self, qidx, p0, p1, scorethld, want_btck=False):
-    res = np.zeros(p1 - p0, DPRES)
-    bt = None
-    if want_btck:
-        lq = int(self.qs.lens()[qidx])
-        d, ln = self.db.dists(), self.db.lens()
-        ncols = int(d[p1 - 1] + ln[p1 - 1] - d[p0])
-        bt = np.zeros((lq, ncols), np.uint8)
-    _check(lib().cth_phase1_block(self.h, qidx, p0, p1, scorethld, res.ctypes.data,
-                                  bt.ctypes.data if bt is not None else None), "cth_phase1_block")
-    return res, bt
-
-
-Context.phase1_block = _phase1_block
-
-
-def _phase2_block(self, qidx, p0, p1, scorethld, searchspace, ssemodel=2):
-    n = p1 - p0
-    res = np.zeros(n, DPRES)
-    plist = np.zeros(n, np.int32)
-    stats = np.zeros((n, 10), np.float64)
-    npass = ctypes.c_int()
-    _check(lib().cth_phase2_block(self.h, qidx, p0, p1, scorethld, searchspace, ssemodel, res.ctypes.data,
-                                  plist.ctypes.data, ctypes.byref(npass), stats.ctypes.data), "cth_phase2_block")
-    k = npass.value
-    return res, plist[:k], stats[:k]
-
-
-Context.phase2_block = _phase2_block
+        Returns (hits HIT array grouped by query, per-query hit counts, SearchStats)."""
+        if q1 is None:
+            q1 = self.qs.nprof
+        if maxhits is None:
+            maxhits = max(1, (q1 - q0) * self.db.nprof)
+        hits = np.zeros(maxhits, HIT)
+        nh = np.zeros(q1 - q0, np.int32)
+        st = SearchStats()
+        opts = SearchOpts(eval, ssemodel, int(do_map), int(timing), mem_budget)
+        _check(lib().cth_search(self.h, q0, q1, ctypes.byref(opts), hits.ctypes.data, maxhits, nh.ctypes.data,
+                                ctypes.byref(st)), "cth_search")
+        return hits[:int(nh.sum())], nh, st

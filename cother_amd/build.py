@@ -14,6 +14,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 LIB = os.path.join(HERE, "libcother_hip.so")
 ARCH = os.environ.get("CTH_OFFLOAD_ARCH", "gfx950")
 
@@ -22,10 +23,10 @@ HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     "-ffp-contract=off", "-fno-fast-math",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-variable", f"-I{INCLUDE}",
 ]
 CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-fno-fast-math", "-Wall",
-             "-I/opt/rocm/include", "-D__HIP_PLATFORM_AMD__"]
+             "-I/opt/rocm/include", "-D__HIP_PLATFORM_AMD__", f"-I{INCLUDE}"]
 
 
 def _run(cmd):

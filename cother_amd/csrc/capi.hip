@@ -12,6 +12,7 @@
 #include <cstring>
 #include <mutex>
 
+#include "cother_hip.h"
 #include "cth_ctx.h"
 #include "cth_dp.h"
 #include "cth_stats.h"
@@ -80,6 +81,8 @@ extern "C" void cth_ctx_destroy(cth_ctx* c) {
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (c->h_pin) (void)hipHostFree(c->h_pin);
+  if (c->has_ev)
+    for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
   if (c->st) (void)hipStreamDestroy(c->st);
   free(c);
 }

@@ -22,6 +22,8 @@ struct cth_ctx {
   void* work; size_t cap_work;
   void* mapbuf; size_t cap_mapbuf;
   void* h_pin; size_t cap_pin;
+  // stage events of cth_search's timing mode (created on first use)
+  hipEvent_t ev[4]; int has_ev;
 };
 
 int cth_ctx_reserve(cth_ctx* c, void** p, size_t* cap, size_t need);

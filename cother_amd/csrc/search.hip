@@ -1,0 +1,225 @@
+// search.hip: the per-query search pipeline over the HBM-resident database.
+//
+// This is the device half of CuBatchProcessing::BatchProcessScoreMatrixDevice
+// (cupro/CuBatchProcessing.cu:230-575) for one query: for each chunk of the
+// resident database whose score buffers fit the memory budget, the score
+// matrix, phase-1 DP and its finalisation run on the context's stream; the
+// passed profiles of the chunk then get their statistics (and, with MAP
+// enabled, the MAP alignment stage).  Host work is limited to the score
+// threshold (JobDispatcher::CalculateScoreThreshold,
+// cualn/JobDispatcher.cpp:806-860), the search-space length adjustment
+// (AbstractScoreMatrix::ComputeLengthAdjustment,
+// libpro/srcsco/AbstractScoreMatrix.cpp:303-395), the passed-profile lists and
+// the final hit ordering.
+//
+// Deterministic by construction: passed profiles are taken in database order
+// (the reference appends them in atomic order, cudp/CuBatchDP_final.cu:260-264)
+// and hits are sorted by (log E-value, database index) (the reference's sort is
+// unstable, cupro/CuBatchProcessingFinalizer.cpp:289-292).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "cth_ctx.h"
+#include "cth_dp.h"
+#include "cth_search.h"
+#include "cth_stats.h"
+
+extern "C" int cth_launch_score(const cth_dpset* q, int qp, const cth_dpset* db, int64_t col0, int ncols,
+                                int64_t ldsc, const cth_tables* dtabs, int lq, float* scores, float* mods,
+                                hipStream_t st);
+extern "C" int cth_launch_dp(const cth_dpset* q, int qp, const cth_dpset* db, int p0, int np, int ltmax,
+                             int64_t col0, int64_t ldsc, const float* scores, uint8_t* btck, float scorethld,
+                             cth_dpres* res, hipStream_t st);
+extern "C" int cth_launch_stats(const cth_dpset* q, int qp, const cth_dpset* db, int p0, int64_t col0, int64_t ldsc,
+                                const float* scores, const int* plist_abs, const int* plist_rel,
+                                const cth_dpres* dpres, int np2, double searchspace, int ssemodel, int* hist,
+                                double* out, hipStream_t st);
+
+namespace {
+
+// JobDispatcher::CalculateScoreThreshold (cualn/JobDispatcher.cpp:806-860)
+void score_threshold(float eval, uint64_t nqyposs, uint64_t dbsize, float* scorethld, float* logevthld) {
+  const uint64_t sspace = nqyposs * dbsize;
+  float ev = eval;
+  if (ev <= 0.0f) { *scorethld = 999.0f; *logevthld = -99.0f; return; }
+  *logevthld = logf(ev);
+  ev *= ev < 1.0f ? 10.0f : 3.0f;
+  ev /= CTH_GAPPED_K * (float)sspace;
+  if (ev <= 0.0f) { *scorethld = 999.0f; return; }
+  const float s = -logf(ev) / CTH_GAPPED_LAMBDA;
+  *scorethld = s < 0.0f ? 0.0f : s;
+}
+
+// AbstractScoreMatrix::ComputeLengthAdjustment (libpro/srcsco/AbstractScoreMatrix.cpp:303-395),
+// Gertz's iteration for the edge-effect length l with l = a/lambda ln(K(m-l)(n-Nl)) + b
+uint64_t length_adjustment(uint64_t m, uint64_t n, uint64_t N) {
+  const float lambda = CTH_GAPPED_LAMBDA, K = CTH_GAPPED_K, alpha = CTH_GAPPED_ALPHA, beta = CTH_GAPPED_BETA;
+  if (N < 1) return m * n;
+  const float logK = logf(K), alratio = alpha / lambda, max_mn = (float)(m > n ? m : n);
+  float length = 0.0f, min_length = 0.0f, max_length = 0.0f, nxt = 0.0f, sspace = 0.0f;
+  bool converged = false;
+  const float a = (float)N, b = (float)(n + m * N), c = (float)(m * n) - max_mn / K;
+  if (c < 0.0f) return m * n;
+  max_length = 2.0f * c / (b + sqrtf(b * b - 4.0f * a * c));
+  for (int j = 0; j < 20; j++) {
+    length = nxt;
+    sspace = (m - length) * (n - N * length);
+    nxt = alratio * (logK + logf(sspace)) + beta;
+    if (length <= nxt) {
+      min_length = length;
+      if (nxt - length <= 1.0f) { converged = true; break; }
+      if (min_length == max_length) break;
+    } else
+      max_length = length;
+    if (nxt < min_length || max_length < nxt) nxt = (!j) ? max_length : (min_length + max_length) * 0.5f;
+  }
+  if (converged) {
+    length = ceilf(min_length);
+    if (length <= max_length) {
+      const float space2 = (m - length) * (n - N * length);
+      nxt = alratio * (logK + logf(space2)) + beta;
+      if (length <= nxt) sspace = space2;
+    }
+  }
+  return (uint64_t)sspace;
+}
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+extern "C" int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt, cth_hit* hits, int maxhits,
+                          int* nhits, cth_search_stats* stats) {
+  if (!c->has_q || !c->has_db) { cth_set_error("search: queries or database not uploaded"); return -1; }
+  if (q0 < 0 || q1 > c->q.d.nprof || q0 >= q1) { cth_set_error("search: bad query range [%d,%d)", q0, q1); return -1; }
+  CTH_HIP_CHECK(hipSetDevice(c->device));
+  const cth_devset& db = c->db;
+  const int nprof = db.d.nprof;
+  const uint64_t dbsize = (uint64_t)db.d.npos;
+  const size_t budget = opt->mem_budget ? opt->mem_budget : ((size_t)16 << 30);
+  const bool timing = opt->timing != 0;
+  if (timing && !c->has_ev) {
+    for (hipEvent_t& e : c->ev) CTH_HIP_CHECK(hipEventCreate(&e));
+    c->has_ev = 1;
+  }
+  int nh = 0;
+  cth_search_stats st{};
+  for (int qi = q0; qi < q1; qi++) {
+    const int lq = c->q.h_len[qi];
+    float scorethld, logethld;
+    score_threshold(opt->eval, (uint64_t)lq, dbsize, &scorethld, &logethld);
+    const double sspace = (double)length_adjustment((uint64_t)lq, dbsize, (uint64_t)nprof);
+    // chunk columns so that scores + modscores + traceback fit the budget
+    const int64_t chunkcols = std::max<int64_t>(4096, (int64_t)(budget / ((size_t)lq * 9)));
+    const int qh0 = nh;
+    for (int p0 = 0; p0 < nprof;) {
+      int p1 = p0;
+      const int64_t col0 = db.h_dist[p0];
+      while (p1 < nprof && (db.h_dist[p1] + db.h_len[p1] - col0 <= chunkcols || p1 == p0)) p1++;
+      const int64_t ncols = db.h_dist[p1 - 1] + db.h_len[p1 - 1] - col0;
+      const int64_t ldsc = (ncols + 63) / 64 * 64;
+      int ltmax = 0;
+      for (int p = p0; p < p1; p++) ltmax = std::max(ltmax, db.h_len[p]);
+      const size_t need = sizeof(float) * (size_t)lq * ldsc;
+      if (cth_ctx_reserve(c, (void**)&c->scores, &c->cap_scores, need)) return -1;
+      if (cth_ctx_reserve(c, (void**)&c->mods, &c->cap_mods, need)) return -1;
+      if (cth_ctx_reserve(c, (void**)&c->btck, &c->cap_btck, (size_t)lq * ldsc)) return -1;
+      if (cth_ctx_reserve(c, (void**)&c->dpout, &c->cap_dpout, sizeof(cth_dpres) * (size_t)(p1 - p0))) return -1;
+      const double t0 = now_ms();
+      if (timing) CTH_HIP_CHECK(hipEventRecord(c->ev[0], c->st));
+      if (cth_launch_score(&c->q.d, qi, &db.d, col0, (int)ncols, ldsc, c->dtab, lq, c->scores, c->mods, c->st))
+        return -1;
+      if (timing) CTH_HIP_CHECK(hipEventRecord(c->ev[1], c->st));
+      if (cth_launch_dp(&c->q.d, qi, &db.d, p0, p1 - p0, ltmax, col0, ldsc, c->scores, c->btck, scorethld,
+                        (cth_dpres*)c->dpout, c->st))
+        return -1;
+      if (timing) CTH_HIP_CHECK(hipEventRecord(c->ev[2], c->st));
+      std::vector<cth_dpres> res(p1 - p0);
+      CTH_HIP_CHECK(hipMemcpyAsync(res.data(), c->dpout, sizeof(cth_dpres) * res.size(), hipMemcpyDeviceToHost,
+                                   c->st));
+      CTH_HIP_CHECK(hipStreamSynchronize(c->st));
+      const double t1 = now_ms();
+      if (timing) {
+        float a = 0.0f, b = 0.0f;
+        CTH_HIP_CHECK(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+        CTH_HIP_CHECK(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+        st.ms_k_score += a;
+        st.ms_k_dp += b;
+      }
+      std::vector<int> rel, abs_;
+      for (int k = 0; k < p1 - p0; k++)
+        if (res[k].passed) { rel.push_back(k); abs_.push_back(p0 + k); }
+      const int np2 = (int)rel.size();
+      std::vector<double> sv((size_t)np2 * CTH_STATS_NOUT);
+      if (np2) {
+        const size_t wneed = sizeof(int) * 2 * np2 + sizeof(int) * 641 * (size_t)np2 +
+                             sizeof(double) * CTH_STATS_NOUT * (size_t)np2 + 256;
+        if (cth_ctx_reserve(c, &c->work, &c->cap_work, wneed)) return -1;
+        int* d_abs = (int*)c->work;
+        int* d_rel = d_abs + np2;
+        int* d_hist = d_rel + np2;
+        double* d_out = (double*)(((uintptr_t)(d_hist + 641 * (size_t)np2) + 15) & ~(uintptr_t)15);
+        CTH_HIP_CHECK(hipMemcpyAsync(d_abs, abs_.data(), sizeof(int) * np2, hipMemcpyHostToDevice, c->st));
+        CTH_HIP_CHECK(hipMemcpyAsync(d_rel, rel.data(), sizeof(int) * np2, hipMemcpyHostToDevice, c->st));
+        if (timing) CTH_HIP_CHECK(hipEventRecord(c->ev[3], c->st));
+        if (cth_launch_stats(&c->q.d, qi, &db.d, p0, col0, ldsc, c->scores, d_abs, d_rel,
+                             (const cth_dpres*)c->dpout, np2, sspace, opt->ssemodel, d_hist, d_out, c->st))
+          return -1;
+        if (timing) CTH_HIP_CHECK(hipEventRecord(c->ev[0], c->st));
+        CTH_HIP_CHECK(hipMemcpyAsync(sv.data(), d_out, sizeof(double) * sv.size(), hipMemcpyDeviceToHost, c->st));
+        CTH_HIP_CHECK(hipStreamSynchronize(c->st));
+        if (timing) {
+          float a = 0.0f;
+          CTH_HIP_CHECK(hipEventElapsedTime(&a, c->ev[3], c->ev[0]));
+          st.ms_k_stats += a;
+        }
+      }
+      const double t2 = now_ms();
+      st.ms_phase1 += t1 - t0;
+      st.ms_phase2 += t2 - t1;
+      st.cells += (double)lq * (double)ncols;
+      st.npassed += np2;
+      st.nchunks += 1;
+      for (int j = 0; j < np2; j++) {
+        const double* s = &sv[(size_t)j * CTH_STATS_NOUT];
+        const float loge = (float)s[CTH_ST_LOGE];
+        // FinalizeMAPDP_MAP drops a profile when logevthld < log E (cumapdp/CuBatchMAPDP_map_final.cu:117-123)
+        if (logethld < loge) continue;
+        if (nh >= maxhits) { cth_set_error("search: hit buffer full (%d)", maxhits); return -1; }
+        cth_hit& h = hits[nh++];
+        memset(&h, 0, sizeof h);
+        const cth_dpres& r = res[rel[j]];
+        h.query = qi;
+        h.profile = abs_[j];
+        h.score = r.score;
+        h.psts = r.psts;
+        h.beg = r.beg;
+        h.end = r.maxxy;
+        h.logE = loge;
+        h.logEpA = (float)s[CTH_ST_LOGEPA];
+        h.lambda_est = (float)s[CTH_ST_LMBDEST];
+        h.K_est = (float)s[CTH_ST_KEST];
+        h.minsc = (float)s[CTH_ST_MIN];
+        h.maxsc = (float)s[CTH_ST_MAX];
+        h.expected = (float)s[CTH_ST_E];
+        h.H = (float)s[CTH_ST_H];
+        h.lambda = (float)s[CTH_ST_LMBD];
+        h.K = (float)s[CTH_ST_K];
+      }
+      p0 = p1;
+    }
+    std::sort(hits + qh0, hits + nh, [](const cth_hit& a, const cth_hit& b) {
+      return a.logE < b.logE || (a.logE == b.logE && a.profile < b.profile);
+    });
+    if (nhits) nhits[qi - q0] = nh - qh0;
+  }
+  if (stats) *stats = st;
+  return 0;
+}

@@ -91,6 +91,57 @@ int cth_phase1_block(cth_ctx* c, int qidx, int p0, int p1, float scorethld, void
 int cth_phase2_block(cth_ctx* c, int qidx, int p0, int p1, float scorethld, double searchspace, int ssemodel,
                      void* res, int* plist, int* npass, double* stats);
 
+/* ---- search over the resident database ------------------------------------
+ * One call runs the whole per-query pipeline of the reference's
+ * CuBatchProcessing::ProcessScoreMatrix (src/libmycu/cupro/
+ * CuBatchProcessing.cu:189-575): for each query, score matrix, phase-1 DP,
+ * finalisation, statistics of the passed profiles and the E-value filter
+ * (JobDispatcher::CalculateScoreThreshold, src/libmycu/cualn/
+ * JobDispatcher.cpp:806-860; AbstractScoreMatrix::ComputeLengthAdjustment,
+ * src/libpro/srcsco/AbstractScoreMatrix.cpp:247-395), database chunked by
+ * mem_budget as the reference chunks by --dev-mem
+ * (CuDeviceMemory::CalcMaxDbDataChunkSizeHelper, src/libmycu/cupro/
+ * CuDeviceMemory.cu:350-470). */
+typedef struct cth_search_opts {
+  float eval;         /* E-value threshold (option EVAL, default 10) */
+  int ssemodel;       /* statistical significance model (SSEMODEL, default 2) */
+  int do_map;         /* realign passed hits by MAP (MAPALN, default 1) */
+  int timing;         /* record HIP events around each stage on the context's stream */
+  uint64_t mem_budget;/* bytes for the per-chunk score/traceback buffers (0: 16 GiB) */
+} cth_search_opts;
+
+/* One reported hit; fields follow the reference's per-profile output record
+ * (TDP2OutputAlnData, src/libmycu/cuss/CuBatchSS_com.h:45-72). */
+typedef struct cth_hit {
+  int32_t query, profile;     /* query index, database profile index */
+  float score;                /* phase-1 score with the correlation bonus */
+  float logE, logEpA;         /* log E-value and log pair E-value */
+  float lambda_est, K_est;    /* estimated gapped lambda and K */
+  float minsc, maxsc, expected, H, lambda, K;
+  int32_t psts;               /* positive substitution scores on the phase-1 path */
+  uint32_t beg, end;          /* phase-1 begin/end coordinates (x<<16|y) */
+  uint32_t mapbeg, mapend;    /* MAP alignment begin/end coordinates */
+  int32_t idnts, ngaps, alnlen, pstvs_map;
+} cth_hit;
+
+typedef struct cth_search_stats {
+  double cells;       /* sum over queries of Lq x database positions */
+  double ms_phase1;   /* wall time of score + DP + finalisation */
+  double ms_phase2;   /* wall time of statistics (+ MAP) */
+  double npassed;     /* profiles passed to phase 2 */
+  double nchunks;     /* database chunks processed (score-kernel launches) */
+  double ms_k_score;  /* with timing: HIP-event time of the score-matrix kernels */
+  double ms_k_dp;     /* with timing: HIP-event time of phase-1 DP + finalisation */
+  double ms_k_stats;  /* with timing: HIP-event time of the statistics kernels */
+} cth_search_stats;
+
+/* Search queries [q0,q1) against the resident database.  hits receives at
+ * most maxhits records, grouped by query in query order, each group sorted by
+ * (logE, profile); nhits[q-q0] the number of hits of query q; stats (may be
+ * NULL) the cells processed and stage times. */
+int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt, cth_hit* hits, int maxhits, int* nhits,
+               cth_search_stats* stats);
+
 /* Lane-layout self-check of the cross-lane move the DP uses (no reference
  * counterpart): out[l] = l-1 for l >= 1, out[0] = -1. */
 int cth_selftest_dpp(int* out64);

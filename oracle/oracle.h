@@ -97,6 +97,22 @@ double orc_lgammaincq(double a, double x);
 void orc_significance(int ssemodel, int lq, int lt, double qeno, double teno, double searchspace, double score,
                       double psts, double lambda, double K, double out[4]);
 
+/* whole search of query qp against database profiles [p0,p1) (orc_search.c);
+ * dbsize/ndb: positions and profiles of the whole database (threshold and
+ * search space); returns the number of hits (sorted by logE, profile) or -1
+ * when maxhits is too small.  Record layout = the product's cth_hit. */
+typedef struct orc_hit {
+  int32_t query, profile;
+  float score, logE, logEpA, lambda_est, K_est;
+  float minsc, maxsc, expected, H, lambda, K;
+  int32_t psts;
+  uint32_t beg, end, mapbeg, mapend;
+  int32_t idnts, ngaps, alnlen, pstvs_map;
+} orc_hit;
+int orc_search_query(const orc_tables* t, const orc_pset* q, int qp, const orc_pset* db, int p0, int p1,
+                     float eval, int ssemodel, uint64_t dbsize, uint64_t ndb, orc_hit* hits, int maxhits,
+                     int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -62,6 +62,10 @@ class Oracle:
         L.orc_histogram.argtypes = [vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int, ctypes.c_int, vp, vp, vp]
         L.orc_length_adjustment.restype = ctypes.c_uint64
         L.orc_length_adjustment.argtypes = [ctypes.c_float] * 4 + [ctypes.c_uint64] * 3 + [ctypes.POINTER(ctypes.c_uint64)]
+        L.orc_search_query.restype = ctypes.c_int
+        L.orc_search_query.argtypes = [ctypes.POINTER(OrcTables), vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_float, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, vp,
+                                       ctypes.c_int, ctypes.c_int]
         self.tables = OrcTables()
         from cother_amd import ENVDIR
         rc = L.orc_tables_load(ENVDIR.encode(), ctypes.byref(self.tables))
@@ -123,6 +127,26 @@ class Oracle:
         dl = ctypes.c_uint64()
         ss = self.L.orc_length_adjustment(lam, K, alpha, beta, m, n, N, ctypes.byref(dl))
         return ss, dl.value
+
+    def search_query(self, q, qp, db, p0=0, p1=None, eval_=10.0, ssemodel=2, dbsize=None, ndb=None,
+                     threads=None):
+        """Hits of query qp against profiles [p0,p1) (HIT records sorted by logE, profile)."""
+        from cother_amd import HIT
+        if p1 is None:
+            p1 = db.nprof
+        if dbsize is None:
+            dbsize = db.npos
+        if ndb is None:
+            ndb = db.nprof
+        if threads is None:
+            threads = min(16, os.cpu_count() or 1)
+        hits = np.zeros(max(1, p1 - p0), HIT)
+        n = self.L.orc_search_query(ctypes.byref(self.tables), ctypes.cast(q.ptr, ctypes.c_void_p), qp,
+                                    ctypes.cast(db.ptr, ctypes.c_void_p), p0, p1, eval_, ssemodel, dbsize, ndb,
+                                    hits.ctypes.data, len(hits), threads)
+        if n < 0:
+            raise RuntimeError("orc_search_query: hit buffer too small")
+        return hits[:n]
 
     def ddms_tab(self, tab22, qrec, trec):
         tab = np.ascontiguousarray(tab22, np.float32)
