@@ -44,3 +44,48 @@ CTH_HD float cth_logf(float x) {
   const float lo = fmaf(fe, 1.42860682e-06f, l1p);
   return fmaf(fe, 0.693145751953125f, lo);
 }
+
+// cth_expf is the exponential the MAP score adjustment uses in place of
+// CUDA's __expf.  Method: k = rint(x / ln2), r = x - k ln2 (ln2 in two
+// parts, HI with 16 significant bits so k*HI is exact for |k| < 256),
+// e^r by its Taylor polynomial to r^7 (|r| <= 0.347: truncation below
+// 5e-9 relative), scaled by 2^k through the exponent bits.  Defined for
+// -87 <= x <= 88; below it returns 0, above it +inf.
+CTH_HD float cth_expf(float x) {
+  if (!(x >= -87.0f)) return x != x ? x : 0.0f;
+  if (x > 88.0f) return INFINITY;
+  const float k = rintf(x * 1.44269504f);
+  float r = fmaf(-k, 0.693145751953125f, x);
+  r = fmaf(-k, 1.42860682e-06f, r);
+  float p = fmaf(r, 1.98412698e-04f, 1.38888889e-03f);  // 1/7!, 1/6!
+  p = fmaf(r, p, 8.33333333e-03f);                     // 1/5!
+  p = fmaf(r, p, 4.16666667e-02f);                     // 1/4!
+  p = fmaf(r, p, 1.66666667e-01f);                     // 1/3!
+  p = fmaf(r, p, 0.5f);
+  p = fmaf(r, p, 1.0f);
+  p = fmaf(r, p, 1.0f);
+  union { float f; uint32_t u; } v;
+  v.u = (uint32_t)((int)k + 127) << 23;
+  return p * v.f;
+}
+
+// Schraudolph's exponential (Neural Computation 11, 1999) with the
+// constants of the reference's MAP kernels (cucom/myfastapproxmath.cuh:
+// 12-41): the float bits int(12102203.16 x + 1064631197) with one fused
+// multiply-add; 0 unless -16 < x (MAPDPfastapprox_expf,
+// cumapdp/CuBatchMAPDP_fwd.cuh:204-214).
+CTH_HD float cth_fastexpf(float x) {
+  if (!(-16.0f < x)) return 0.0f;
+  union { float f; int32_t i; } v;
+  v.i = (int32_t)fmaf(12102203.16f, x, 1064631197.0f);
+  return v.f;
+}
+
+// Its inverse: (bits(x) - 1064631197) * 8.2629582959339e-08; -99 unless
+// 3.8e-11 < x (MAPDPfastapprox_logf, cumapdp/CuBatchMAPDP_fwd.cuh:234-240).
+CTH_HD float cth_fastlogf(float x) {
+  if (!(3.8e-11f < x)) return -99.0f;
+  union { float f; int32_t i; } v;
+  v.f = x;
+  return (float)(v.i - 1064631197) * 8.2629582959339e-08f;
+}
