@@ -9,13 +9,15 @@ synthetic profiles in the reference's value conventions.
 
 A step is one query searched against the whole resident database: score
 matrix (log-odds + SS + CVS + DDMS), phase-1 DP with traceback and
-finalisation, statistics and E-values of the passed profiles, E-value filter
-and hit ordering.  Step s uses query s mod 64.  Cells = Lq x database
+finalisation, statistics and E-values of the passed profiles, E-value filter,
+MAP realignment (forward, backward, MAP DP, traceback) of the significant
+hits and hit ordering.  Step s uses query s mod 64.  Cells = Lq x database
 positions; value = cells of the timed steps / wall time of the timed steps.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -117,6 +119,8 @@ def main():
     ap.add_argument("--eval", type=float, default=10.0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample size in seconds (0: skip)")
     ap.add_argument("--seed", type=int, default=20260)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run the plain copy kernel over BYTES (traffic-counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the search path shards over GPUs by database chunk "
@@ -125,6 +129,14 @@ def main():
         sys.exit(2)
 
     import cother_amd
+    if args.copy_calibration:
+        ms = ctypes.c_float()
+        rc = cother_amd.lib().cth_bench_copy(0, args.copy_calibration, 3, ctypes.byref(ms))
+        if rc:
+            raise RuntimeError(cother_amd.lib().cth_last_error().decode())
+        print(json.dumps({"copy_bytes": args.copy_calibration, "launches": 4, "ms_per_launch": ms.value,
+                          "GBps": 2 * args.copy_calibration / (ms.value * 1e-3) / 1e9}), flush=True)
+        return
     threads = min(16, os.cpu_count() or 1)
     t0 = time.perf_counter()
     qs = cother_amd.synth(args.queries, 1, 250.0, 0.35, 40, 1000, dist_mode=0, sort_desc=0, seed=args.seed,
@@ -149,6 +161,7 @@ def main():
         log(f"warmup step {s} (query {qi}, Lq={ql[qi]}) done")
 
     lat, cells, ops, ms_score, nlaunch, ms_dp, ms_st, npass = [], 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0
+    ms_map, map_cells, nhits = 0.0, 0.0, 0
     t_all = time.perf_counter()
     for s in range(args.steps):
         qi = (args.warmup + s) % qs.nprof
@@ -159,12 +172,16 @@ def main():
         ms_score += st.ms_k_score
         ms_dp += st.ms_k_dp
         ms_st += st.ms_k_stats
+        ms_map += st.ms_k_map
+        map_cells += st.map_cells
+        nhits += int(nh[0])
         nlaunch += st.nchunks
         npass += st.npassed
         qh = np.bincount(qdd[qd[qi]:qd[qi] + ql[qi]], minlength=128)[:128].astype(np.float64)
         ops += algorithmic_ops(qh, db_hist, ql[qi], db.npos, U)
         log(f"step {s}: query {qi} Lq={ql[qi]} {lat[-1]:.0f} ms, {int(nh[0])} hits, {int(st.npassed)} passed, "
-            f"score {st.ms_k_score:.0f} ms dp {st.ms_k_dp:.0f} ms stats {st.ms_k_stats:.0f} ms")
+            f"score {st.ms_k_score:.0f} ms dp {st.ms_k_dp:.0f} ms stats {st.ms_k_stats:.0f} ms "
+            f"map {st.ms_k_map:.0f} ms")
     wall = time.perf_counter() - t_all
     gcups = cells / wall / 1e9
 
@@ -189,10 +206,12 @@ def main():
                                "(lognormal median 220, 30-1500); one query per step",
                    "queries": qs.nprof, "templates": db.nprof, "db_positions": int(db.npos),
                    "query_positions_timed": int(sum(ql[(args.warmup + s) % qs.nprof] for s in range(args.steps))),
-                   "eval": args.eval, "map": False},
+                   "eval": args.eval, "map": True},
         "p50_ms": round(float(np.percentile(lat_a, 50)), 2), "p99_ms": round(float(np.percentile(lat_a, 99)), 2),
-        "stage_ms": {"score": round(ms_score, 1), "dp": round(ms_dp, 1), "stats": round(ms_st, 1)},
-        "passed_per_query": round(npass / args.steps, 1),
+        "stage_ms": {"score": round(ms_score, 1), "dp": round(ms_dp, 1), "stats": round(ms_st, 1),
+                     "map": round(ms_map, 1)},
+        "passed_per_query": round(npass / args.steps, 1), "hits_per_query": round(nhits / args.steps, 1),
+        "map_cells": map_cells,
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -41,13 +41,14 @@ class CthPset(ctypes.Structure):
 class SearchOpts(ctypes.Structure):
     """Mirror of cth_search_opts (include/cother_hip.h)."""
     _fields_ = [("eval", ctypes.c_float), ("ssemodel", ctypes.c_int), ("do_map", ctypes.c_int),
-                ("timing", ctypes.c_int), ("mem_budget", ctypes.c_uint64)]
+                ("timing", ctypes.c_int), ("mem_budget", ctypes.c_uint64), ("minpp", ctypes.c_float),
+                ("reserved", ctypes.c_int)]
 
 
 class SearchStats(ctypes.Structure):
     """Mirror of cth_search_stats (include/cother_hip.h)."""
     _fields_ = [(n, ctypes.c_double) for n in ("cells", "ms_phase1", "ms_phase2", "npassed", "nchunks",
-                                               "ms_k_score", "ms_k_dp", "ms_k_stats")]
+                                               "ms_k_score", "ms_k_dp", "ms_k_stats", "ms_k_map", "map_cells")]
 
     def asdict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -59,10 +60,16 @@ HIT = np.dtype([("query", np.int32), ("profile", np.int32), ("score", np.float32
                 ("maxsc", np.float32), ("expected", np.float32), ("H", np.float32), ("lambda_", np.float32),
                 ("K", np.float32), ("psts", np.int32), ("beg", np.uint32), ("end", np.uint32),
                 ("mapbeg", np.uint32), ("mapend", np.uint32), ("idnts", np.int32), ("ngaps", np.int32),
-                ("alnlen", np.int32), ("pstvs_map", np.int32)])
+                ("alnlen", np.int32), ("pstvs_map", np.int32), ("mapscore", np.float32)])
 
 DPRES = np.dtype([("maxsc", np.float32), ("maxxy", np.uint32), ("passed", np.int32), ("score", np.float32),
                   ("psts", np.int32), ("beg", np.uint32)])
+
+# Mirror of struct cth_mapres (cother_amd/csrc/cth_map.h)
+MAPRES = np.dtype([("maxsc", np.float32), ("end", np.uint32), ("beg", np.uint32), ("idnts", np.int32),
+                   ("psts", np.int32), ("ngaps", np.int32), ("alnlen", np.int32), ("logz", np.float32),
+                   ("maxlog", np.float32)])
+MINPP = 0.28
 
 _lib = None
 
@@ -97,6 +104,10 @@ def lib():
     L.cth_phase2_block.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                                    ctypes.c_double, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                    ctypes.POINTER(ctypes.c_int), ctypes.c_void_p]
+    L.cth_map_block.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_void_p]
+    L.cth_selftest_dpp_shl.argtypes = [ctypes.c_void_p]
+    L.cth_bench_copy.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
     L.cth_search.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(SearchOpts),
                              ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(SearchStats)]
     _lib = L
@@ -215,8 +226,17 @@ class Context:
         k = npass.value
         return res, plist[:k], stats[:k]
 
-    def search(self, q0=0, q1=None, eval=10.0, ssemodel=2, do_map=False, timing=False, mem_budget=0,
-               maxhits=None):
+    def map_block(self, qidx, p0, p1, profiles, logepa, minpp=MINPP):
+        """MAP stage records (MAPRES) of query qidx for the given profiles of block [p0,p1)."""
+        prof = np.ascontiguousarray(profiles, np.int32)
+        epa = np.ascontiguousarray(logepa, np.float32)
+        res = np.zeros(len(prof), MAPRES)
+        _check(lib().cth_map_block(self.h, qidx, p0, p1, prof.ctypes.data, epa.ctypes.data, len(prof), minpp,
+                                   res.ctypes.data), "cth_map_block")
+        return res
+
+    def search(self, q0=0, q1=None, eval=10.0, ssemodel=2, do_map=True, timing=False, mem_budget=0,
+               maxhits=None, minpp=MINPP):
         """Search queries [q0,q1) against the resident database.
 
         Returns (hits HIT array grouped by query, per-query hit counts, SearchStats)."""
@@ -227,7 +247,7 @@ class Context:
         hits = np.zeros(maxhits, HIT)
         nh = np.zeros(q1 - q0, np.int32)
         st = SearchStats()
-        opts = SearchOpts(eval, ssemodel, int(do_map), int(timing), mem_budget)
+        opts = SearchOpts(eval, ssemodel, int(do_map), int(timing), mem_budget, minpp, 0)
         _check(lib().cth_search(self.h, q0, q1, ctypes.byref(opts), hits.ctypes.data, maxhits, nh.ctypes.data,
                                 ctypes.byref(st)), "cth_search")
         return hits[:int(nh.sum())], nh, st

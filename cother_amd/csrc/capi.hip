@@ -15,6 +15,7 @@
 #include "cother_hip.h"
 #include "cth_ctx.h"
 #include "cth_dp.h"
+#include "cth_map.h"
 #include "cth_stats.h"
 #include <vector>
 
@@ -211,5 +212,94 @@ extern "C" int cth_phase2_block(cth_ctx* c, int qidx, int p0, int p1, float scor
     return -1;
   CTH_HIP_CHECK(hipStreamSynchronize(c->st));
   CTH_HIP_CHECK(hipMemcpy(stats, d_out, sizeof(double) * 10 * np2, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int cth_launch_map(const cth_dpset* q, int qp, int lq, const cth_dpset* db, const cth_mapjob* jobs,
+                              int njobs, int ltmax, int64_t col0, int64_t ldsc, const float* scores,
+                              const float* mods, float* post, uint8_t* btck, double* zrow, float minpp,
+                              cth_mapres* res, hipStream_t st);
+
+// Score matrix of query qidx against template profiles [p0,p1), then the MAP
+// stage for njobs of them (absolute indices profiles[], log pair E-values
+// logepa[]); res receives njobs records of cth_mapres.
+extern "C" int cth_map_block(cth_ctx* c, int qidx, int p0, int p1, const int32_t* profiles, const float* logepa,
+                             int njobs, float minpp, void* res) {
+  if (!c->has_q || !c->has_db) { cth_set_error("map_block: queries or database not uploaded"); return -1; }
+  if (qidx < 0 || qidx >= c->q.d.nprof) { cth_set_error("map_block: query %d out of range", qidx); return -1; }
+  if (p0 < 0 || p1 > c->db.d.nprof || p0 >= p1) { cth_set_error("map_block: bad profile range [%d,%d)", p0, p1); return -1; }
+  if (njobs <= 0) return 0;
+  std::vector<cth_mapjob> jobs(njobs);
+  std::vector<char> seen(p1 - p0, 0);
+  int ltmax = 0;
+  for (int j = 0; j < njobs; j++) {
+    if (profiles[j] < p0 || profiles[j] >= p1) { cth_set_error("map_block: profile %d outside the block", profiles[j]); return -1; }
+    // each job writes its profile's columns of the posterior buffer: no duplicates
+    if (seen[profiles[j] - p0]++) { cth_set_error("map_block: profile %d listed twice", profiles[j]); return -1; }
+    jobs[j].profile = profiles[j];
+    jobs[j].logepa = logepa[j];
+    ltmax = c->db.h_len[profiles[j]] > ltmax ? c->db.h_len[profiles[j]] : ltmax;
+  }
+  CTH_HIP_CHECK(hipSetDevice(c->device));
+  const int lq = c->q.h_len[qidx];
+  const int64_t col0 = c->db.h_dist[p0];
+  const int64_t ncols = c->db.h_dist[p1 - 1] + c->db.h_len[p1 - 1] - col0;
+  if (ncols > (1ll << 30)) { cth_set_error("map_block: too many columns"); return -1; }
+  const int64_t ldsc = (ncols + 63) / 64 * 64;
+  const size_t need = sizeof(float) * (size_t)lq * ldsc;
+  if (cth_ctx_reserve(c, (void**)&c->scores, &c->cap_scores, need)) return -1;
+  if (cth_ctx_reserve(c, (void**)&c->mods, &c->cap_mods, need)) return -1;
+  if (cth_ctx_reserve(c, (void**)&c->mapbuf, &c->cap_mapbuf, need)) return -1;
+  if (cth_ctx_reserve(c, (void**)&c->btck, &c->cap_btck, (size_t)lq * ldsc)) return -1;
+  const size_t wneed = sizeof(double) * (size_t)njobs * lq + sizeof(cth_mapjob) * njobs +
+                       sizeof(cth_mapres) * njobs + 512;
+  if (cth_ctx_reserve(c, &c->work, &c->cap_work, wneed)) return -1;
+  double* d_zrow = (double*)c->work;
+  cth_mapjob* d_jobs = (cth_mapjob*)(d_zrow + (size_t)njobs * lq);
+  cth_mapres* d_res = (cth_mapres*)(((uintptr_t)(d_jobs + njobs) + 15) & ~(uintptr_t)15);
+  CTH_HIP_CHECK(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(cth_mapjob) * njobs, hipMemcpyHostToDevice, c->st));
+  if (cth_launch_score(&c->q.d, qidx, &c->db.d, col0, (int)ncols, ldsc, c->dtab, lq, c->scores, c->mods, c->st))
+    return -1;
+  if (cth_launch_map(&c->q.d, qidx, lq, &c->db.d, d_jobs, njobs, ltmax, col0, ldsc, c->scores, c->mods,
+                     (float*)c->mapbuf, c->btck, d_zrow, minpp, d_res, c->st))
+    return -1;
+  CTH_HIP_CHECK(hipMemcpyAsync(res, d_res, sizeof(cth_mapres) * njobs, hipMemcpyDeviceToHost, c->st));
+  CTH_HIP_CHECK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+namespace {
+__global__ void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+}  // namespace
+
+// Measurement aid (no reference counterpart): copies `bytes` (a multiple of
+// 16) between two device buffers `reps` times with a plain streaming kernel,
+// so that HBM traffic counters can be scaled against a kernel of known
+// traffic (bytes read and written per launch); *ms receives the mean time.
+extern "C" int cth_bench_copy(int device, size_t bytes, int reps, float* ms) {
+  if (bytes == 0 || bytes % 16 || reps < 1) { cth_set_error("bench_copy: bad size"); return -1; }
+  CTH_HIP_CHECK(hipSetDevice(device));
+  void *a = nullptr, *b = nullptr;
+  CTH_HIP_CHECK(hipMalloc(&a, bytes));
+  if (hipMalloc(&b, bytes) != hipSuccess) { (void)hipFree(a); cth_set_error("bench_copy: out of memory"); return -1; }
+  hipEvent_t e0, e1;
+  CTH_HIP_CHECK(hipEventCreate(&e0));
+  CTH_HIP_CHECK(hipEventCreate(&e1));
+  const size_t n = bytes / 16;
+  hipLaunchKernelGGL(k_copy16, dim3(4096), dim3(256), 0, 0, (const uint4*)a, (uint4*)b, n);
+  CTH_HIP_CHECK(hipEventRecord(e0, 0));
+  for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy16, dim3(4096), dim3(256), 0, 0, (const uint4*)a, (uint4*)b, n);
+  CTH_HIP_CHECK(hipEventRecord(e1, 0));
+  CTH_HIP_CHECK(hipEventSynchronize(e1));
+  float t = 0.0f;
+  CTH_HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
+  if (ms) *ms = t / reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  CTH_HIP_CHECK(hipFree(a));
+  CTH_HIP_CHECK(hipFree(b));
   return 0;
 }

@@ -20,7 +20,7 @@ struct cth_ctx {
   uint8_t* btck; size_t cap_btck;
   float* dpout; size_t cap_dpout;
   void* work; size_t cap_work;
-  void* mapbuf; size_t cap_mapbuf;
+  float* mapbuf; size_t cap_mapbuf;  // MAP posteriors [Lq][ldsc]
   void* h_pin; size_t cap_pin;
   // stage events of cth_search's timing mode (created on first use)
   hipEvent_t ev[4]; int has_ev;

@@ -26,6 +26,7 @@
 
 #include "cth_ctx.h"
 #include "cth_dp.h"
+#include "cth_map.h"
 #include "cth_search.h"
 #include "cth_stats.h"
 
@@ -39,6 +40,10 @@ extern "C" int cth_launch_stats(const cth_dpset* q, int qp, const cth_dpset* db,
                                 const float* scores, const int* plist_abs, const int* plist_rel,
                                 const cth_dpres* dpres, int np2, double searchspace, int ssemodel, int* hist,
                                 double* out, hipStream_t st);
+extern "C" int cth_launch_map(const cth_dpset* q, int qp, int lq, const cth_dpset* db, const cth_mapjob* jobs,
+                              int njobs, int ltmax, int64_t col0, int64_t ldsc, const float* scores,
+                              const float* mods, float* post, uint8_t* btck, double* zrow, float minpp,
+                              cth_mapres* res, hipStream_t st);
 
 namespace {
 
@@ -116,8 +121,9 @@ extern "C" int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt
     float scorethld, logethld;
     score_threshold(opt->eval, (uint64_t)lq, dbsize, &scorethld, &logethld);
     const double sspace = (double)length_adjustment((uint64_t)lq, dbsize, (uint64_t)nprof);
-    // chunk columns so that scores + modscores + traceback fit the budget
-    const int64_t chunkcols = std::max<int64_t>(4096, (int64_t)(budget / ((size_t)lq * 9)));
+    // chunk columns so that scores + modscores + traceback (+ MAP posteriors) fit the budget
+    const size_t bytes_per_cell = opt->do_map ? 13 : 9;
+    const int64_t chunkcols = std::max<int64_t>(4096, (int64_t)(budget / ((size_t)lq * bytes_per_cell)));
     const int qh0 = nh;
     for (int p0 = 0; p0 < nprof;) {
       int p1 = p0;
@@ -181,17 +187,55 @@ extern "C" int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt
           st.ms_k_stats += a;
         }
       }
+      // MAP stage for the passed profiles within the E-value threshold (the
+      // reference's MAP kernels return early for the others,
+      // cumapdp/CuBatchMAPDP_fwd.cu:127-130)
+      std::vector<int> sel;
+      for (int j = 0; j < np2; j++)
+        if (!(logethld < (float)sv[(size_t)j * CTH_STATS_NOUT + CTH_ST_LOGE])) sel.push_back(j);
+      std::vector<cth_mapres> mres(sel.size());
+      if (opt->do_map && !sel.empty()) {
+        const int nj = (int)sel.size();
+        std::vector<cth_mapjob> jobs(nj);
+        int ltm = 0;
+        for (int k = 0; k < nj; k++) {
+          jobs[k].profile = abs_[sel[k]];
+          jobs[k].logepa = (float)sv[(size_t)sel[k] * CTH_STATS_NOUT + CTH_ST_LOGEPA];
+          ltm = std::max(ltm, db.h_len[jobs[k].profile]);
+        }
+        if (cth_ctx_reserve(c, (void**)&c->mapbuf, &c->cap_mapbuf, need)) return -1;
+        const size_t wneed = sizeof(double) * (size_t)nj * lq + sizeof(cth_mapjob) * nj + sizeof(cth_mapres) * nj + 512;
+        if (cth_ctx_reserve(c, &c->work, &c->cap_work, wneed)) return -1;
+        double* d_zrow = (double*)c->work;
+        cth_mapjob* d_jobs = (cth_mapjob*)(d_zrow + (size_t)nj * lq);
+        cth_mapres* d_res = (cth_mapres*)(((uintptr_t)(d_jobs + nj) + 15) & ~(uintptr_t)15);
+        CTH_HIP_CHECK(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(cth_mapjob) * nj, hipMemcpyHostToDevice, c->st));
+        if (timing) CTH_HIP_CHECK(hipEventRecord(c->ev[3], c->st));
+        if (cth_launch_map(&c->q.d, qi, lq, &db.d, d_jobs, nj, ltm, col0, ldsc, c->scores, c->mods, c->mapbuf,
+                           c->btck, d_zrow, opt->minpp > 0.0f ? opt->minpp : CTH_MINPP, d_res, c->st))
+          return -1;
+        if (timing) CTH_HIP_CHECK(hipEventRecord(c->ev[0], c->st));
+        CTH_HIP_CHECK(hipMemcpyAsync(mres.data(), d_res, sizeof(cth_mapres) * nj, hipMemcpyDeviceToHost, c->st));
+        CTH_HIP_CHECK(hipStreamSynchronize(c->st));
+        if (timing) {
+          float a = 0.0f;
+          CTH_HIP_CHECK(hipEventElapsedTime(&a, c->ev[3], c->ev[0]));
+          st.ms_k_map += a;
+        }
+        for (int k = 0; k < nj; k++) st.map_cells += (double)lq * (double)db.h_len[jobs[k].profile];
+      }
       const double t2 = now_ms();
       st.ms_phase1 += t1 - t0;
       st.ms_phase2 += t2 - t1;
       st.cells += (double)lq * (double)ncols;
       st.npassed += np2;
       st.nchunks += 1;
-      for (int j = 0; j < np2; j++) {
+      for (size_t k = 0; k < sel.size(); k++) {
+        const int j = sel[k];
         const double* s = &sv[(size_t)j * CTH_STATS_NOUT];
-        const float loge = (float)s[CTH_ST_LOGE];
-        // FinalizeMAPDP_MAP drops a profile when logevthld < log E (cumapdp/CuBatchMAPDP_map_final.cu:117-123)
-        if (logethld < loge) continue;
+        // the host keeps hits with an alignment (CompressResultsPlain,
+        // cupro/CuBatchProcessingFinalizerPlain.cpp:87-142)
+        if (opt->do_map && mres[k].alnlen < 1) continue;
         if (nh >= maxhits) { cth_set_error("search: hit buffer full (%d)", maxhits); return -1; }
         cth_hit& h = hits[nh++];
         memset(&h, 0, sizeof h);
@@ -202,7 +246,7 @@ extern "C" int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt
         h.psts = r.psts;
         h.beg = r.beg;
         h.end = r.maxxy;
-        h.logE = loge;
+        h.logE = (float)s[CTH_ST_LOGE];
         h.logEpA = (float)s[CTH_ST_LOGEPA];
         h.lambda_est = (float)s[CTH_ST_LMBDEST];
         h.K_est = (float)s[CTH_ST_KEST];
@@ -212,6 +256,15 @@ extern "C" int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt
         h.H = (float)s[CTH_ST_H];
         h.lambda = (float)s[CTH_ST_LMBD];
         h.K = (float)s[CTH_ST_K];
+        if (opt->do_map) {
+          h.mapbeg = mres[k].beg;
+          h.mapend = mres[k].end;
+          h.idnts = mres[k].idnts;
+          h.ngaps = mres[k].ngaps;
+          h.alnlen = mres[k].alnlen;
+          h.pstvs_map = mres[k].psts;
+          h.mapscore = mres[k].maxsc;
+        }
       }
       p0 = p1;
     }

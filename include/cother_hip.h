@@ -18,6 +18,7 @@
 #ifndef COTHER_HIP_H
 #define COTHER_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -91,6 +92,16 @@ int cth_phase1_block(cth_ctx* c, int qidx, int p0, int p1, float scorethld, void
 int cth_phase2_block(cth_ctx* c, int qidx, int p0, int p1, float scorethld, double searchspace, int ssemodel,
                      void* res, int* plist, int* npass, double* stats);
 
+/* Score matrix of query qidx against template profiles [p0,p1), then the
+ * MAP alignment stage (CuBatchMAPDP::PerformCompleteMAPDynProgDevice,
+ * src/libmycu/cumapdp/CuBatchMAPDP.cu:67-140) for njobs of those profiles
+ * (absolute indices profiles[], log pair E-values logepa[] as phase 2
+ * computed them; minpp the MINPP option, default 0.28).  res receives njobs
+ * records of nine 32-bit words {maxsc(f32), end(u32), beg(u32), idnts,
+ * psts, ngaps, alnlen (i32), logz(f32), maxlog(f32)}. */
+int cth_map_block(cth_ctx* c, int qidx, int p0, int p1, const int32_t* profiles, const float* logepa, int njobs,
+                  float minpp, void* res);
+
 /* ---- search over the resident database ------------------------------------
  * One call runs the whole per-query pipeline of the reference's
  * CuBatchProcessing::ProcessScoreMatrix (src/libmycu/cupro/
@@ -108,6 +119,8 @@ typedef struct cth_search_opts {
   int do_map;         /* realign passed hits by MAP (MAPALN, default 1) */
   int timing;         /* record HIP events around each stage on the context's stream */
   uint64_t mem_budget;/* bytes for the per-chunk score/traceback buffers (0: 16 GiB) */
+  float minpp;        /* MAP posterior threshold (MINPP; 0: the default 0.28) */
+  int reserved;
 } cth_search_opts;
 
 /* One reported hit; fields follow the reference's per-profile output record
@@ -121,7 +134,8 @@ typedef struct cth_hit {
   int32_t psts;               /* positive substitution scores on the phase-1 path */
   uint32_t beg, end;          /* phase-1 begin/end coordinates (x<<16|y) */
   uint32_t mapbeg, mapend;    /* MAP alignment begin/end coordinates */
-  int32_t idnts, ngaps, alnlen, pstvs_map;
+  int32_t idnts, ngaps, alnlen, pstvs_map;  /* identities, gaps, length, positives of the MAP alignment */
+  float mapscore;             /* MAP alignment score */
 } cth_hit;
 
 typedef struct cth_search_stats {
@@ -133,6 +147,8 @@ typedef struct cth_search_stats {
   double ms_k_score;  /* with timing: HIP-event time of the score-matrix kernels */
   double ms_k_dp;     /* with timing: HIP-event time of phase-1 DP + finalisation */
   double ms_k_stats;  /* with timing: HIP-event time of the statistics kernels */
+  double ms_k_map;    /* with timing: HIP-event time of the MAP kernels */
+  double map_cells;   /* sum over realigned hits of Lq x Lt */
 } cth_search_stats;
 
 /* Search queries [q0,q1) against the resident database.  hits receives at
@@ -142,9 +158,17 @@ typedef struct cth_search_stats {
 int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt, cth_hit* hits, int maxhits, int* nhits,
                cth_search_stats* stats);
 
-/* Lane-layout self-check of the cross-lane move the DP uses (no reference
- * counterpart): out[l] = l-1 for l >= 1, out[0] = -1. */
+/* Measurement aid (no reference counterpart): a plain device-to-device copy
+ * kernel over `bytes` (multiple of 16), launched `reps` times; *ms receives
+ * the mean launch time.  Its HBM traffic is known (bytes read + bytes
+ * written), which calibrates the traffic counters of the search kernels. */
+int cth_bench_copy(int device, size_t bytes, int reps, float* ms);
+
+/* Lane-layout self-checks of the cross-lane moves the DP kernels use (no
+ * reference counterpart): shr: out[l] = l-1 for l >= 1, out[0] = -1;
+ * shl: out[l] = l+1 for l <= 62, out[63] = -1. */
 int cth_selftest_dpp(int* out64);
+int cth_selftest_dpp_shl(int* out64);
 
 #ifdef __cplusplus
 }

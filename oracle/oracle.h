@@ -129,9 +129,10 @@ typedef struct orc_hit {
   int32_t psts;
   uint32_t beg, end, mapbeg, mapend;
   int32_t idnts, ngaps, alnlen, pstvs_map;
+  float mapscore;
 } orc_hit;
 int orc_search_query(const orc_tables* t, const orc_pset* q, int qp, const orc_pset* db, int p0, int p1,
-                     float eval, int ssemodel, uint64_t dbsize, uint64_t ndb, orc_hit* hits, int maxhits,
+                     float eval, int ssemodel, int do_map, uint64_t dbsize, uint64_t ndb, orc_hit* hits, int maxhits,
                      int nthreads);
 
 #ifdef __cplusplus

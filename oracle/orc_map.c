@@ -35,9 +35,10 @@
  * The maximum log forward probability is max(0, max over the matrix of the
  * forward MM), which is what the reference's per-column running maximum
  * reduces to (the block runs start from 0, out-of-matrix cells are below
- * it).  Z is accumulated in double precision in row-major order: the
+ * it).  Z is accumulated in double precision in a fixed order (each row
+ * from its last column to its first, then the row sums from the top): the
  * reference sums float partial sums with atomicAdd in launch order
- * (nondeterministic), so Z agrees to rounding only.  The alignment maximum is
+ * (nondeterministic), so Z agrees with it to rounding only.  The alignment maximum is
  * the first maximum in row-major order over the valid area (the reference's
  * winner among exactly equal maxima depends on its block schedule).
  */
@@ -213,9 +214,14 @@ void orc_map_bwd(const orc_pset* q, int qp, const orc_pset* db, int tp, const fl
   float* dtr = (float*)malloc(sizeof(float) * 7 * (size_t)W);
   for (int c = 0; c <= lt; c++) trn_bwd(db, tp, c, lt, dtr + 7 * c);
   for (int k = 0; k < NST * W; k++) nxt[k] = LP0;
+  /* Z in a fixed order: per row over columns from right to left, then the
+   * row sums from the top row down (all in double) */
   double z = 0.0;
-  for (int y = 0; y < lq; y++)
-    for (int c = 0; c < lt; c++) z += (double)orc_fastexpf(fwd[(int64_t)y * lt + c] - maxlog);
+  for (int y = 0; y < lq; y++) {
+    double rs = 0.0;
+    for (int c = lt - 1; c >= 0; c--) rs += (double)orc_fastexpf(fwd[(int64_t)y * lt + c] - maxlog);
+    z += rs;
+  }
   *zsum = z;
   for (int y = lq - 1; y >= 0; y--) {
     float qt[7];

@@ -4,9 +4,10 @@
  *
  * Pipeline as CuBatchProcessing::BatchProcessScoreMatrixDevice runs it
  * (src/libmycu/cupro/CuBatchProcessing.cu:230-575): score matrix, phase-1 DP
- * and finalisation, statistics of the passed profiles, then the E-value
+ * and finalisation, statistics of the passed profiles, the E-value
  * filter of FinalizeMAPDP_MAP (src/libmycu/cumapdp/CuBatchMAPDP_map_final.cu:
- * 117-123).  Each profile is processed on its own (its score matrix only),
+ * 117-123) and, with do_map, the MAP stage (hits without a MAP alignment are
+ * dropped as the host finalizer drops them).  Each profile is processed on its own (its score matrix only),
  * so memory stays at one Lq x Lt block per thread.
  */
 #include <math.h>
@@ -29,7 +30,7 @@ static int hit_cmp(const void* a, const void* b) {
 }
 
 int orc_search_query(const orc_tables* t, const orc_pset* q, int qp, const orc_pset* db, int p0, int p1,
-                     float eval, int ssemodel, uint64_t dbsize, uint64_t ndb, orc_hit* hits, int maxhits,
+                     float eval, int ssemodel, int do_map, uint64_t dbsize, uint64_t ndb, orc_hit* hits, int maxhits,
                      int nthreads) {
   const int lq = q->len[qp];
   float scorethld, logethld;
@@ -72,6 +73,12 @@ int orc_search_query(const orc_tables* t, const orc_pset* q, int qp, const orc_p
       orc_significance(ssemodel, lq, lt, q->eno[qp], db->eno[p], sspace, score, psts, par[4], par[5], sig);
       const float loge = (float)sig[2];
       if (logethld < loge) continue;
+      orc_mapres mr;
+      memset(&mr, 0, sizeof mr);
+      if (do_map) {
+        orc_map_profile(q, qp, db, p, sc, md, lt, 0, (float)sig[3], 0.28f, &mr, NULL, 0);
+        if (mr.alnlen < 1) continue; /* hits need an alignment (CuBatchProcessingFinalizerPlain.cpp:87-142) */
+      }
       orc_hit* h = &all[k];
       h->query = qp;
       h->profile = p;
@@ -89,6 +96,15 @@ int orc_search_query(const orc_tables* t, const orc_pset* q, int qp, const orc_p
       h->psts = psts;
       h->beg = beg;
       h->end = maxxy;
+      if (do_map) {
+        h->mapbeg = mr.beg;
+        h->mapend = mr.end;
+        h->idnts = mr.idnts;
+        h->ngaps = mr.ngaps;
+        h->alnlen = mr.alnlen;
+        h->pstvs_map = mr.psts;
+        h->mapscore = mr.maxsc;
+      }
       keep[k] = 1;
     }
     free(sc); free(md); free(bt);

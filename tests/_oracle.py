@@ -64,8 +64,10 @@ class Oracle:
         L.orc_length_adjustment.argtypes = [ctypes.c_float] * 4 + [ctypes.c_uint64] * 3 + [ctypes.POINTER(ctypes.c_uint64)]
         L.orc_search_query.restype = ctypes.c_int
         L.orc_search_query.argtypes = [ctypes.POINTER(OrcTables), vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int,
-                                       ctypes.c_float, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, vp,
-                                       ctypes.c_int, ctypes.c_int]
+                                       ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64,
+                                       vp, ctypes.c_int, ctypes.c_int]
+        L.orc_map_profile.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int, vp, vp, ctypes.c_int64, ctypes.c_int64,
+                                      ctypes.c_float, ctypes.c_float, vp, vp, ctypes.c_int]
         self.tables = OrcTables()
         from cother_amd import ENVDIR
         rc = L.orc_tables_load(ENVDIR.encode(), ctypes.byref(self.tables))
@@ -129,7 +131,7 @@ class Oracle:
         return ss, dl.value
 
     def search_query(self, q, qp, db, p0=0, p1=None, eval_=10.0, ssemodel=2, dbsize=None, ndb=None,
-                     threads=None):
+                     threads=None, do_map=True):
         """Hits of query qp against profiles [p0,p1) (HIT records sorted by logE, profile)."""
         from cother_amd import HIT
         if p1 is None:
@@ -142,11 +144,30 @@ class Oracle:
             threads = min(16, os.cpu_count() or 1)
         hits = np.zeros(max(1, p1 - p0), HIT)
         n = self.L.orc_search_query(ctypes.byref(self.tables), ctypes.cast(q.ptr, ctypes.c_void_p), qp,
-                                    ctypes.cast(db.ptr, ctypes.c_void_p), p0, p1, eval_, ssemodel, dbsize, ndb,
+                                    ctypes.cast(db.ptr, ctypes.c_void_p), p0, p1, eval_, ssemodel, int(do_map), dbsize, ndb,
                                     hits.ctypes.data, len(hits), threads)
         if n < 0:
             raise RuntimeError("orc_search_query: hit buffer too small")
         return hits[:n]
+
+    def map_profile(self, q, qp, db, p, sc, md, col, logepa, minpp=0.28, want_aln=False):
+        """MAP stage of query qp vs profile p on the score/modular-score block
+        sc, md ([Lq][ncols], profile p at column col): a MAPRES record (and the
+        query/middle/target alignment strings)."""
+        from cother_amd import MAPRES
+        res = np.zeros(1, MAPRES)
+        cap = int(q.lens()[qp]) + int(db.lens()[p]) + 2
+        aln = ctypes.create_string_buffer(3 * cap) if want_aln else None
+        s_ = np.ascontiguousarray(sc, np.float32)
+        m_ = np.ascontiguousarray(md, np.float32)
+        self.L.orc_map_profile(ctypes.cast(q.ptr, ctypes.c_void_p), qp, ctypes.cast(db.ptr, ctypes.c_void_p), p,
+                               s_.ctypes.data, m_.ctypes.data, s_.shape[1], col, logepa, minpp, res.ctypes.data,
+                               aln, cap)
+        if not want_aln:
+            return res[0]
+        n = int(res[0]["alnlen"])
+        raw = aln.raw
+        return res[0], (raw[:n].decode(), raw[cap:cap + n].decode(), raw[2 * cap:2 * cap + n].decode())
 
     def ddms_tab(self, tab22, qrec, trec):
         tab = np.ascontiguousarray(tab22, np.float32)

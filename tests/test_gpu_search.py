@@ -7,7 +7,8 @@ import cother_amd
 
 pytestmark = pytest.mark.gpu
 
-FIELDS = ("profile", "score", "psts", "beg", "end")
+FIELDS = ("profile", "score", "psts", "beg", "end", "mapbeg", "mapend", "idnts", "ngaps", "alnlen", "pstvs_map",
+          "mapscore")
 STATS = ("logE", "logEpA", "lambda_est", "K_est", "minsc", "maxsc", "expected", "H", "lambda_", "K")
 
 

@@ -13,7 +13,7 @@ def test_search_composition_matches_stages(oracle):
     q = cother_amd.synth(1, 0, lmin=50, lmax=90, dist_mode=0, sort_desc=0, seed=7)
     db = cother_amd.synth(30, 0, lmin=20, lmax=120, dist_mode=1, sort_desc=1, seed=8)
     eval_ = 1e4  # permissive: keeps many hits so the ordering is exercised
-    hits = oracle.search_query(q, 0, db, eval_=eval_)
+    hits = oracle.search_query(q, 0, db, eval_=eval_, do_map=False)
     lq = int(q.lens()[0])
     thld, logethld = oracle.score_threshold(eval_, lq, db.npos)
     sspace, _ = oracle.length_adjustment(lq, db.npos, db.nprof)
@@ -45,8 +45,23 @@ def test_search_composition_matches_stages(oracle):
 def test_search_strict_threshold_filters(oracle):
     q = cother_amd.synth(1, 0, lmin=50, lmax=90, dist_mode=0, sort_desc=0, seed=7)
     db = cother_amd.synth(30, 0, lmin=20, lmax=120, dist_mode=1, sort_desc=1, seed=8)
-    loose = oracle.search_query(q, 0, db, eval_=1e4)
-    strict = oracle.search_query(q, 0, db, eval_=1e-3)
+    loose = oracle.search_query(q, 0, db, eval_=1e4, do_map=False)
+    strict = oracle.search_query(q, 0, db, eval_=1e-3, do_map=False)
     assert len(strict) <= len(loose)
     assert set(strict["profile"]) <= set(loose["profile"])
     assert (strict["logE"] <= np.float32(np.log(np.float32(1e-3)))).all()
+
+
+def test_search_with_map_keeps_aligned_hits(oracle):
+    q = cother_amd.synth(1, 0, lmin=50, lmax=90, dist_mode=0, sort_desc=0, seed=7)
+    db = cother_amd.synth(30, 0, lmin=20, lmax=120, dist_mode=1, sort_desc=1, seed=8)
+    plain = oracle.search_query(q, 0, db, eval_=1e4, do_map=False)
+    mapped = oracle.search_query(q, 0, db, eval_=1e4, do_map=True)
+    # MAP drops hits without an alignment and changes nothing else
+    assert set(mapped["profile"]) <= set(plain["profile"])
+    assert (mapped["alnlen"] >= 1).all()
+    byp = {int(h["profile"]): h for h in plain}
+    for h in mapped:
+        w = byp[int(h["profile"])]
+        assert h["logE"] == w["logE"] and h["score"] == w["score"]
+        assert h["idnts"] + h["pstvs_map"] + h["ngaps"] <= h["alnlen"]
