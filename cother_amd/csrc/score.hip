@@ -83,63 +83,80 @@ __device__ __forceinline__ uint32_t mkc(int qb, int qe, int pb, int pe) {
 // step 2; per diagonal a Kadane scan in steps of 4 cells where cells 2 and 4
 // are exact pair scores and cells 1 and 3 their means; per 4-cell step the
 // best qualifying segment (length >= DSTSEGM) goes to the top-3 register.
+//
+// Branch-free form of the reference's per-cell update: a segment's begin
+// index is kept on one axis only (the other is that index plus the diagonal
+// offset); "score > 0 and better than the step's best" reduces to
+// "better than the step's best" because the best starts at 0; and the top-3
+// register is entered only when the step's best can displace its smallest
+// entry (the register stays ordered m0 >= m1 >= m2 >= 0).  The clamp at 0
+// is a max: it differs from the reference's select only in the sign of a zero
+// sum, and a zero is never registered.  The query list is
+// the same for the whole wavefront, so its values are read into scalar
+// registers.
+#define CTH_DDMS_UPD(K, SK, IDX)                                               \
+  {                                                                           \
+    const int ik = (IDX) + (K);                                               \
+    segb = (segsco == 0.0f) ? ik : segb;                                      \
+    segsco = fmaxf(segsco + (SK), 0.0f);                                      \
+    const bool take = (scom < segsco) & (segb <= ik + 1 - CTH_DSTSEGM);       \
+    scom = take ? segsco : scom;                                              \
+    bb = take ? segb : bb;                                                    \
+    be = take ? ik : be;                                                      \
+  }
+
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
 __device__ float ddms_cell(const float* __restrict__ dmsT, const uint8_t* __restrict__ q,
                            const uint8_t* __restrict__ t) {
-  const int len1 = q[0], len2 = t[0];
-  const int dstsegm = CTH_DSTSEGM;
-  if (len1 < dstsegm || len2 < dstsegm) return 0.0f;
+  const int len1 = uni(q[0]), len2 = t[0];
+  if (len1 < CTH_DSTSEGM || len2 < CTH_DSTSEGM) return 0.0f;
   float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f;
   uint32_t c0 = 0xffffffffu, c1 = 0xffffffffu, c2 = 0xffffffffu;
+  // upper part: diagonal j pairs query index i with template index j + i;
+  // segb is the query index where the running segment begins
   for (int j = len2 >> 2; 0 <= j; j -= 2) {
-    int segqb = 0, segpb = j;
-    float s0 = dmsT[q[1] * 22 + t[1 + j]];
+    const uint8_t* tj = t + 1 + j;
+    int segb = 0;
+    float s0 = dmsT[uni(q[1]) * 22 + tj[0]];
     float segsco = s0 < 0.0f ? 0.0f : s0;
-    for (int i = 0; i < len1 && j + i < len2; i += 4) {
-      float sc[5];
-      sc[0] = s0;
-      sc[2] = dmsT[q[1 + i + 2] * 22 + t[1 + j + i + 2]];
-      sc[4] = dmsT[q[1 + i + 4] * 22 + t[1 + j + i + 4]];
-      sc[3] = 0.5f * (sc[2] + sc[4]);
-      sc[1] = 0.5f * (sc[0] + sc[2]);
+    const int iend = min(len1, len2 - j);
+    for (int i = 0; i < iend; i += 4) {
+      const float s2 = dmsT[uni(q[1 + i + 2]) * 22 + tj[i + 2]];
+      const float s4 = dmsT[uni(q[1 + i + 4]) * 22 + tj[i + 4]];
+      const float s3 = 0.5f * (s2 + s4);
+      const float s1 = 0.5f * (s0 + s2);
       float scom = 0.0f;
-      int im = 0, ib = 0, jib = 0;
-#pragma unroll
-      for (int k = 1; k <= 4; k++) {
-        if (!segsco) { segqb = i + k; segpb = j + i + k; }
-        const float x = segsco + sc[k];
-        segsco = x < 0.0f ? 0.0f : x;
-        if (segsco != 0.0f && scom < segsco && dstsegm <= i + k - segqb + 1) {
-          scom = segsco; im = i + k; ib = segqb; jib = segpb;
-        }
-      }
-      register_max(m0, m1, m2, c0, c1, c2, scom, mkc(ib, im, jib, j + im));
-      s0 = sc[4];
+      int bb = 0, be = 0;
+      CTH_DDMS_UPD(1, s1, i)
+      CTH_DDMS_UPD(2, s2, i)
+      CTH_DDMS_UPD(3, s3, i)
+      CTH_DDMS_UPD(4, s4, i)
+      if (scom != 0.0f && m2 <= scom) register_max(m0, m1, m2, c0, c1, c2, scom, mkc(bb, be, bb + j, be + j));
+      s0 = s4;
     }
   }
+  // lower part: diagonal i pairs query index i + j with template index j;
+  // segb is the template index where the running segment begins
   for (int i = len1 >> 2; 0 < i; i -= 2) {
-    int segqb = i, segpb = 0;
-    float s0 = dmsT[q[1 + i] * 22 + t[1]];
+    const uint8_t* qi = q + 1 + i;
+    int segb = 0;
+    float s0 = dmsT[uni(qi[0]) * 22 + t[1]];
     float segsco = s0 < 0.0f ? 0.0f : s0;
-    for (int j = 0; j < len2 && i + j < len1; j += 4) {
-      float sc[5];
-      sc[0] = s0;
-      sc[2] = dmsT[q[1 + i + j + 2] * 22 + t[1 + j + 2]];
-      sc[4] = dmsT[q[1 + i + j + 4] * 22 + t[1 + j + 4]];
-      sc[3] = 0.5f * (sc[2] + sc[4]);
-      sc[1] = 0.5f * (sc[0] + sc[2]);
+    const int jend = min(len2, len1 - i);
+    for (int j = 0; j < jend; j += 4) {
+      const float s2 = dmsT[uni(qi[j + 2]) * 22 + t[1 + j + 2]];
+      const float s4 = dmsT[uni(qi[j + 4]) * 22 + t[1 + j + 4]];
+      const float s3 = 0.5f * (s2 + s4);
+      const float s1 = 0.5f * (s0 + s2);
       float scom = 0.0f;
-      int jm = 0, ijb = 0, jb = 0;
-#pragma unroll
-      for (int k = 1; k <= 4; k++) {
-        if (!segsco) { segqb = i + j + k; segpb = j + k; }
-        const float x = segsco + sc[k];
-        segsco = x < 0.0f ? 0.0f : x;
-        if (segsco != 0.0f && scom < segsco && dstsegm <= j + k - segpb + 1) {
-          scom = segsco; jm = j + k; ijb = segqb; jb = segpb;
-        }
-      }
-      register_max(m0, m1, m2, c0, c1, c2, scom, mkc(ijb, i + jm, jb, jm));
-      s0 = sc[4];
+      int bb = 0, be = 0;
+      CTH_DDMS_UPD(1, s1, j)
+      CTH_DDMS_UPD(2, s2, j)
+      CTH_DDMS_UPD(3, s3, j)
+      CTH_DDMS_UPD(4, s4, j)
+      if (scom != 0.0f && m2 <= scom) register_max(m0, m1, m2, c0, c1, c2, scom, mkc(bb + i, be + i, bb, be));
+      s0 = s4;
     }
   }
   float s = 0.0f;
@@ -149,6 +166,7 @@ __device__ float ddms_cell(const float* __restrict__ dmsT, const uint8_t* __rest
   s = fmaf(CTH_DDMS_SLOPE * CTH_DDMS_SCALE, s, CTH_DDMS_ICEPT);
   return s * CTH_DDMSWGT;
 }
+#undef CTH_DDMS_UPD
 
 __global__ __launch_bounds__(256) void k_score(cth_dpset q, int qp, cth_dpset db, int64_t col0, int ncols,
                                                int64_t ldsc, const cth_tables* __restrict__ tabs,
