@@ -74,7 +74,7 @@ def algorithmic_ops(qlens_hist, db_hist, lq, npos, U):
 def read_traffic():
     """HBM bytes per score cell of the score kernel from the committed rocprofv3
     --pmc summary (line 'bytes_per_cell: X'), or None."""
-    p = os.path.join(ROOT, "profiles", "k_score_c1_pmc.txt")
+    p = os.path.join(ROOT, "profiles", "k_score_db6000_pmc.txt")
     if not os.path.exists(p):
         return None
     for line in open(p):

@@ -30,7 +30,7 @@ constexpr int kListPitch = 132; // LDS bytes per template list
 
 struct ScoreLds {
   uint8_t tl[kCols * kListPitch];
-  uint8_t ql[kRows][128];
+  int32_t qo[kRows][128];  // query lists: entry 0 = length, then 88 * value
   float dms[22 * 22];
   float ss[467];
   float cvvals[CTH_CVMAXCARD];
@@ -105,11 +105,17 @@ __device__ __forceinline__ uint32_t mkc(int qb, int qe, int pb, int pe) {
     be = take ? ik : be;                                                      \
   }
 
-__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+// the pair score of list values whose table byte offsets are qb (query,
+// d1*88) and tv (template value d2): dmsT[d1][d2]
+__device__ __forceinline__ float pair(const float* __restrict__ dmsT, int qb, int tv) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(dmsT) + qb + (tv << 2));
+}
 
-__device__ float ddms_cell(const float* __restrict__ dmsT, const uint8_t* __restrict__ q,
+// q: the query list as table byte offsets (entry k = 88 * value k, entry 0 =
+// the list length); t: the template list (byte 0 = length, then values)
+__device__ float ddms_cell(const float* __restrict__ dmsT, const int32_t* __restrict__ q,
                            const uint8_t* __restrict__ t) {
-  const int len1 = uni(q[0]), len2 = t[0];
+  const int len1 = q[0], len2 = t[0];
   if (len1 < CTH_DSTSEGM || len2 < CTH_DSTSEGM) return 0.0f;
   float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f;
   uint32_t c0 = 0xffffffffu, c1 = 0xffffffffu, c2 = 0xffffffffu;
@@ -118,12 +124,12 @@ __device__ float ddms_cell(const float* __restrict__ dmsT, const uint8_t* __rest
   for (int j = len2 >> 2; 0 <= j; j -= 2) {
     const uint8_t* tj = t + 1 + j;
     int segb = 0;
-    float s0 = dmsT[uni(q[1]) * 22 + tj[0]];
+    float s0 = pair(dmsT, q[1], tj[0]);
     float segsco = s0 < 0.0f ? 0.0f : s0;
     const int iend = min(len1, len2 - j);
     for (int i = 0; i < iend; i += 4) {
-      const float s2 = dmsT[uni(q[1 + i + 2]) * 22 + tj[i + 2]];
-      const float s4 = dmsT[uni(q[1 + i + 4]) * 22 + tj[i + 4]];
+      const float s2 = pair(dmsT, q[1 + i + 2], tj[i + 2]);
+      const float s4 = pair(dmsT, q[1 + i + 4], tj[i + 4]);
       const float s3 = 0.5f * (s2 + s4);
       const float s1 = 0.5f * (s0 + s2);
       float scom = 0.0f;
@@ -139,14 +145,14 @@ __device__ float ddms_cell(const float* __restrict__ dmsT, const uint8_t* __rest
   // lower part: diagonal i pairs query index i + j with template index j;
   // segb is the template index where the running segment begins
   for (int i = len1 >> 2; 0 < i; i -= 2) {
-    const uint8_t* qi = q + 1 + i;
+    const int32_t* qi = q + 1 + i;
     int segb = 0;
-    float s0 = dmsT[uni(qi[0]) * 22 + t[1]];
+    float s0 = pair(dmsT, qi[0], t[1]);
     float segsco = s0 < 0.0f ? 0.0f : s0;
     const int jend = min(len2, len1 - i);
     for (int j = 0; j < jend; j += 4) {
-      const float s2 = dmsT[uni(qi[j + 2]) * 22 + t[1 + j + 2]];
-      const float s4 = dmsT[uni(qi[j + 4]) * 22 + t[1 + j + 4]];
+      const float s2 = pair(dmsT, qi[j + 2], t[1 + j + 2]);
+      const float s4 = pair(dmsT, qi[j + 4], t[1 + j + 4]);
       const float s3 = 0.5f * (s2 + s4);
       const float s1 = 0.5f * (s0 + s2);
       float scom = 0.0f;
@@ -192,12 +198,12 @@ __global__ __launch_bounds__(256) void k_score(cth_dpset q, int qp, cth_dpset db
     if (cc >= ncols && d == 0) v = 0x15151500u;  // empty list
     *reinterpret_cast<uint32_t*>(&L.tl[c * kListPitch + d * 4]) = v;
   }
-  for (int k = tid; k < kRows * 32; k += 256) {
-    const int r = k >> 5, d = k & 31;
+  for (int k = tid; k < kRows * 128; k += 256) {
+    const int r = k >> 7, d = k & 127;
     const int rr = blockIdx.y * kRows + r;
-    uint32_t v = 0x15151500u;
-    if (rr < lq) v = reinterpret_cast<const uint32_t*>(q.dd + (size_t)(q0 + rr) * CTH_DDSTRIDE)[d];
-    reinterpret_cast<uint32_t*>(L.ql[r])[d] = v;
+    int v = (d == 0) ? 0 : CTH_DD_MISSING;
+    if (rr < lq) v = q.dd[(size_t)(q0 + rr) * CTH_DDSTRIDE + d];
+    L.qo[r][d] = (d == 0) ? v : v * 22 * (int)sizeof(float);
   }
   __syncthreads();
   if (row >= lq || col >= ncols) return;
@@ -261,7 +267,7 @@ __global__ __launch_bounds__(256) void k_score(cth_dpset q, int qp, cth_dpset db
   s = s + sc;
   float ms = fmaf(-CTH_CVSSHIFT, CTH_CVSWGT, sc);
 
-  const float d = ddms_cell(L.dms, L.ql[w], &L.tl[lane * kListPitch]);
+  const float d = ddms_cell(L.dms, L.qo[w], &L.tl[lane * kListPitch]);
   if (d != 0.0f) {
     s = s + d;
     ms = ms + fmaf(-CTH_DDMSSHIFT, CTH_DDMSWGT, d);
