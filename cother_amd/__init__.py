@@ -42,7 +42,7 @@ class SearchOpts(ctypes.Structure):
     """Mirror of cth_search_opts (include/cother_hip.h)."""
     _fields_ = [("eval", ctypes.c_float), ("ssemodel", ctypes.c_int), ("do_map", ctypes.c_int),
                 ("timing", ctypes.c_int), ("mem_budget", ctypes.c_uint64), ("minpp", ctypes.c_float),
-                ("reserved", ctypes.c_int)]
+                ("reserved", ctypes.c_int), ("alnbuf", ctypes.c_void_p), ("alncap", ctypes.c_uint64)]
 
 
 class SearchStats(ctypes.Structure):
@@ -60,7 +60,8 @@ HIT = np.dtype([("query", np.int32), ("profile", np.int32), ("score", np.float32
                 ("maxsc", np.float32), ("expected", np.float32), ("H", np.float32), ("lambda_", np.float32),
                 ("K", np.float32), ("psts", np.int32), ("beg", np.uint32), ("end", np.uint32),
                 ("mapbeg", np.uint32), ("mapend", np.uint32), ("idnts", np.int32), ("ngaps", np.int32),
-                ("alnlen", np.int32), ("pstvs_map", np.int32), ("mapscore", np.float32)])
+                ("alnlen", np.int32), ("pstvs_map", np.int32), ("mapscore", np.float32), ("reserved", np.int32),
+                ("alnoff", np.uint64)])
 
 DPRES = np.dtype([("maxsc", np.float32), ("maxxy", np.uint32), ("passed", np.int32), ("score", np.float32),
                   ("psts", np.int32), ("beg", np.uint32)])
@@ -166,6 +167,19 @@ def synth(nprof, len_mode, median=0.0, sigma=0.0, lmin=0, lmax=0, dist_mode=1, s
                                      threads))
 
 
+def concat(sets):
+    """One profile set holding the profiles of `sets` in order."""
+    lens = np.concatenate([s_.lens() for s_ in sets]).astype(np.int32)
+    out = Pset(lib().cth_pset_alloc(len(lens), lens.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+    for name in ("trg", "ss", "cv", "trn"):
+        out.field(name)[...] = np.concatenate([s_.field(name) for s_ in sets], axis=1)
+    for name in ("cvprior", "cvnorm2", "eno", "aa", "sss"):
+        out.field(name)[...] = np.concatenate([s_.field(name) for s_ in sets])
+    for name in ("bkg", "dd"):
+        out.field(name)[...] = np.concatenate([s_.field(name) for s_ in sets], axis=0)
+    return out
+
+
 class Context:
     """A device context (cth_ctx): tables, uploaded queries, resident DB."""
 
@@ -236,10 +250,12 @@ class Context:
         return res
 
     def search(self, q0=0, q1=None, eval=10.0, ssemodel=2, do_map=True, timing=False, mem_budget=0,
-               maxhits=None, minpp=MINPP):
+               maxhits=None, minpp=MINPP, alignments=False):
         """Search queries [q0,q1) against the resident database.
 
-        Returns (hits HIT array grouped by query, per-query hit counts, SearchStats)."""
+        Returns (hits HIT array grouped by query, per-query hit counts, SearchStats), and with
+        alignments=True also a list of five-row alignments (query, middle, target, query SS,
+        target SS) per hit."""
         if q1 is None:
             q1 = self.qs.nprof
         if maxhits is None:
@@ -247,7 +263,21 @@ class Context:
         hits = np.zeros(maxhits, HIT)
         nh = np.zeros(q1 - q0, np.int32)
         st = SearchStats()
-        opts = SearchOpts(eval, ssemodel, int(do_map), int(timing), mem_budget, minpp, 0)
+        abuf = None
+        if alignments:
+            cap = 5 * min(maxhits, 4096) * int(self.qs.lens()[q0:q1].max() + self.db.lens().max())
+            abuf = ctypes.create_string_buffer(cap)
+        opts = SearchOpts(eval, ssemodel, int(do_map), int(timing), mem_budget, minpp, 0,
+                          ctypes.cast(abuf, ctypes.c_void_p) if abuf is not None else None,
+                          len(abuf) if abuf is not None else 0)
         _check(lib().cth_search(self.h, q0, q1, ctypes.byref(opts), hits.ctypes.data, maxhits, nh.ctypes.data,
                                 ctypes.byref(st)), "cth_search")
-        return hits[:int(nh.sum())], nh, st
+        hits = hits[:int(nh.sum())]
+        if not alignments:
+            return hits, nh, st
+        raw = abuf.raw
+        alns = []
+        for h in hits:
+            n, o = int(h["alnlen"]), int(h["alnoff"])
+            alns.append(tuple(raw[o + r * n:o + (r + 1) * n].decode() for r in range(5)))
+        return hits, nh, st, alns

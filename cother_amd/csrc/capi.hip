@@ -218,7 +218,7 @@ extern "C" int cth_phase2_block(cth_ctx* c, int qidx, int p0, int p1, float scor
 extern "C" int cth_launch_map(const cth_dpset* q, int qp, int lq, const cth_dpset* db, const cth_mapjob* jobs,
                               int njobs, int ltmax, int64_t col0, int64_t ldsc, const float* scores,
                               const float* mods, float* post, uint8_t* btck, double* zrow, float minpp,
-                              cth_mapres* res, hipStream_t st);
+                              char* alnout, cth_mapres* res, hipStream_t st);
 
 // Score matrix of query qidx against template profiles [p0,p1), then the MAP
 // stage for njobs of them (absolute indices profiles[], log pair E-values
@@ -261,7 +261,7 @@ extern "C" int cth_map_block(cth_ctx* c, int qidx, int p0, int p1, const int32_t
   if (cth_launch_score(&c->q.d, qidx, &c->db.d, col0, (int)ncols, ldsc, c->dtab, lq, c->scores, c->mods, c->st))
     return -1;
   if (cth_launch_map(&c->q.d, qidx, lq, &c->db.d, d_jobs, njobs, ltmax, col0, ldsc, c->scores, c->mods,
-                     (float*)c->mapbuf, c->btck, d_zrow, minpp, d_res, c->st))
+                     (float*)c->mapbuf, c->btck, d_zrow, minpp, nullptr, d_res, c->st))
     return -1;
   CTH_HIP_CHECK(hipMemcpyAsync(res, d_res, sizeof(cth_mapres) * njobs, hipMemcpyDeviceToHost, c->st));
   CTH_HIP_CHECK(hipStreamSynchronize(c->st));

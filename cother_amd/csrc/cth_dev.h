@@ -25,6 +25,7 @@ struct cth_dpset {
   const float*   eno;      // [nprof]
   const float*   bkg;      // [nprof*20]
   const char*    aa;       // [ld]
+  const char*    sss;      // [ld] SS state letters (0: the profile has no SS prediction)
   const int32_t* pro;      // [ld] profile index of each position
   const float*   trg;      // [20*ld]
   const float*   trn;      // [7*ldtr]

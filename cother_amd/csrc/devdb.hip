@@ -42,7 +42,7 @@ int up_fields(void** dst, const float* src, int nf, int64_t n, int64_t ld) {
 
 extern "C" void cth_devset_free(cth_devset* s) {
   if (!s) return;
-  for (void* p : {(void*)s->d.len, (void*)s->d.dist, (void*)s->d.eno, (void*)s->d.bkg, (void*)s->d.aa,
+  for (void* p : {(void*)s->d.len, (void*)s->d.dist, (void*)s->d.eno, (void*)s->d.bkg, (void*)s->d.aa, (void*)s->d.sss,
                   (void*)s->d.pro, (void*)s->d.trg, (void*)s->d.trn, (void*)s->d.ss, (void*)s->d.cv,
                   (void*)s->d.cvprior, (void*)s->d.cvnorm2, (void*)s->d.dd})
     if (p) (void)hipFree(p);
@@ -67,6 +67,7 @@ extern "C" int cth_devset_upload(const cth_pset* h, cth_devset* s) {
   rc |= up((void**)&d.eno, h->eno, h->nprof, np);
   rc |= up((void**)&d.bkg, h->bkg, (size_t)h->nprof * CTH_NAA, (size_t)np * CTH_NAA);
   rc |= up((void**)&d.aa, h->aa, n, ld);
+  rc |= up((void**)&d.sss, h->sss, n, ld);
   {
     std::vector<int32_t> pro(ld, 0);
     for (int p = 0; p < h->nprof; p++)

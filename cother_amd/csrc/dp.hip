@@ -47,12 +47,13 @@ __device__ __forceinline__ bool in_valid_area(int lq, int lt, int edge, int y, i
   return !corners && 0 <= y && y < lq && 0 <= x && x < lt;
 }
 
-__global__ __launch_bounds__(64) void k_dp(cth_dpset q, int qp, cth_dpset db, int pbase, int64_t col0, int64_t ldsc,
+__global__ __launch_bounds__(64) void k_dp(cth_dpset q, int qp, cth_dpset db, int pbase,
+                                           const int* __restrict__ plist, int64_t col0, int64_t ldsc,
                                            const float* __restrict__ scores, uint8_t* __restrict__ btck,
                                            cth_dpres* __restrict__ res) {
   extern __shared__ float bot[];  // [5][ltmax]: bottom row of the previous stripe
   const int lane = threadIdx.x;
-  const int p = pbase + blockIdx.x;
+  const int p = plist ? plist[blockIdx.x] : pbase + blockIdx.x;
   const int lq = q.len[qp], lt = db.len[p];
   const int64_t colp = db.dist[p] - col0;
   const int64_t qb = q.dist[qp] + qp;      // query transition vector 0
@@ -153,7 +154,7 @@ __global__ __launch_bounds__(64) void k_dp(cth_dpset q, int qp, cth_dpset db, in
         cMM = nMM; cMI = nMI; cIM = nIM; cDN = nDN; cND = nND;
         if (0 <= c) {
           if (best < nMM) { best = nMM; bestxy = ((uint32_t)c << 16) | (uint32_t)y; }
-          btck[(int64_t)y * ldsc + colp + c] = (uint8_t)bk;
+          if (btck) btck[(int64_t)y * ldsc + colp + c] = (uint8_t)bk;
         }
       }
       pMM = uMM; pMI = uMI; pIM = uIM; pDN = uDN; pND = uND;
@@ -179,13 +180,14 @@ __global__ __launch_bounds__(64) void k_dp(cth_dpset q, int qp, cth_dpset db, in
 }
 
 // lane 0 of each wavefront finalises one profile
-__global__ __launch_bounds__(64) void k_dpfin(cth_dpset db, int pbase, int64_t col0, int64_t ldsc,
-                                              const float* __restrict__ scores, const uint8_t* __restrict__ btck,
-                                              float scorethld, cth_dpres* __restrict__ res) {
+__global__ __launch_bounds__(64) void k_dpfin(cth_dpset db, int pbase, const int* __restrict__ plist, int64_t col0,
+                                              int64_t ldsc, const float* __restrict__ scores,
+                                              const uint8_t* __restrict__ btck, float scorethld,
+                                              cth_dpres* __restrict__ res) {
   __shared__ float cache[36];
   __shared__ float tree[32];
   if (threadIdx.x != 0) return;
-  const int p = pbase + blockIdx.x;
+  const int p = plist ? plist[blockIdx.x] : pbase + blockIdx.x;
   const int64_t colp = db.dist[p] - col0;
   cth_dpres r = res[blockIdx.x];
   r.passed = 0; r.score = 0.0f; r.psts = 0; r.beg = 0;
@@ -251,15 +253,28 @@ extern "C" int cth_selftest_dpp(int* host_out64) {
   return 0;
 }
 
+// Phase 1 for np profiles: p0 + k, or plist[k] (device array of absolute
+// indices) when plist is not NULL; res[k] receives profile k's record.
+// btck NULL: no traceback bytes (maximum and its coordinates only);
+// finalize: run the finalisation (needs btck).
+extern "C" int cth_launch_dp_list(const cth_dpset* q, int qp, const cth_dpset* db, int p0, const int* plist, int np,
+                                  int ltmax, int64_t col0, int64_t ldsc, const float* scores, uint8_t* btck,
+                                  float scorethld, int finalize, cth_dpres* res, hipStream_t st) {
+  if (np <= 0) return 0;
+  if (finalize && !btck) { cth_set_error("dp: finalisation needs the traceback"); return -1; }
+  const size_t lds = sizeof(float) * 5 * (size_t)(ltmax > 0 ? ltmax : 1);
+  if (lds > 160 * 1024) { cth_set_error("dp: profile length %d exceeds the LDS budget", ltmax); return -1; }
+  hipLaunchKernelGGL(k_dp, dim3(np), dim3(64), lds, st, *q, qp, *db, p0, plist, col0, ldsc, scores, btck, res);
+  CTH_HIP_CHECK(hipGetLastError());
+  if (finalize) {
+    hipLaunchKernelGGL(k_dpfin, dim3(np), dim3(64), 0, st, *db, p0, plist, col0, ldsc, scores, btck, scorethld, res);
+    CTH_HIP_CHECK(hipGetLastError());
+  }
+  return 0;
+}
+
 extern "C" int cth_launch_dp(const cth_dpset* q, int qp, const cth_dpset* db, int p0, int np, int ltmax,
                              int64_t col0, int64_t ldsc, const float* scores, uint8_t* btck, float scorethld,
                              cth_dpres* res, hipStream_t st) {
-  if (np <= 0) return 0;
-  const size_t lds = sizeof(float) * 5 * (size_t)(ltmax > 0 ? ltmax : 1);
-  if (lds > 160 * 1024) { cth_set_error("dp: profile length %d exceeds the LDS budget", ltmax); return -1; }
-  hipLaunchKernelGGL(k_dp, dim3(np), dim3(64), lds, st, *q, qp, *db, p0, col0, ldsc, scores, btck, res);
-  CTH_HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_dpfin, dim3(np), dim3(64), 0, st, *db, p0, col0, ldsc, scores, btck, scorethld, res);
-  CTH_HIP_CHECK(hipGetLastError());
-  return 0;
+  return cth_launch_dp_list(q, qp, db, p0, nullptr, np, ltmax, col0, ldsc, scores, btck, scorethld, 1, res, st);
 }

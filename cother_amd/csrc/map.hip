@@ -335,14 +335,23 @@ __global__ __launch_bounds__(64) void k_map_align(cth_dpset q, int qp, cth_dpset
 
 // FinalizeMAPDP_MAP traceback: from the maximum (from (0,0) when it is not
 // positive), identities by residue letter, positives by original score > 0.
+// With alnout, the five alignment rows of the reference's output
+// (cumapdp/CuBatchMAPDP_map_final.cu:228-300: query, middle, target, query
+// SS, target SS) go to alnout + job*5*rowcap, row r at r*rowcap, in traceback
+// order (from the end of the alignment).
 __global__ __launch_bounds__(64) void k_map_fin(cth_dpset q, int qp, cth_dpset db, const cth_mapjob* __restrict__ jobs,
                                                 int64_t col0, int64_t ldsc, const float* __restrict__ scores,
-                                                const uint8_t* __restrict__ btck, cth_mapres* __restrict__ res) {
+                                                const uint8_t* __restrict__ btck, char* __restrict__ alnout,
+                                                int rowcap, cth_mapres* __restrict__ res) {
   if (threadIdx.x != 0) return;
   const int p = jobs[blockIdx.x].profile;
   const int64_t colp = db.dist[p] - col0;
   const char* qaa = q.aa + q.dist[qp];
   const char* taa = db.aa + db.dist[p];
+  const char* qss = q.sss + q.dist[qp];
+  const char* tss = db.sss + db.dist[p];
+  const bool qssok = qss[0] != 0, tssok = tss[0] != 0;
+  char* o = alnout ? alnout + (size_t)blockIdx.x * 5 * rowcap : nullptr;
   cth_mapres r = res[blockIdx.x];
   const uint32_t xx = (0.0f < r.maxsc) ? r.end : 0u;
   int y = (int)(xx & 0xffff), x = (int)(xx >> 16);
@@ -351,12 +360,26 @@ __global__ __launch_bounds__(64) void k_map_fin(cth_dpset q, int qp, cth_dpset d
     const int64_t a = (int64_t)y * ldsc + colp + x;
     const int b = btck[a];
     if (b == CTH_BT_STOP) break;
-    if (b == CTH_BT_UP) { gaps++; y--; }
-    else if (b == CTH_BT_LEFT) { gaps++; x--; }
-    else {
-      if (qaa[y] == taa[x]) idts++;
-      else if (scores[a] > 0.0f) psts++;
+    char cq, cm, ct, sq = ' ', st = ' ';
+    if (b == CTH_BT_UP) {
+      gaps++; cq = qaa[y]; ct = '-'; cm = ' ';
+      if (qssok) sq = qss[y];
+      y--;
+    } else if (b == CTH_BT_LEFT) {
+      gaps++; cq = '-'; ct = taa[x]; cm = ' ';
+      if (tssok) st = tss[x];
+      x--;
+    } else {
+      cq = qaa[y]; ct = taa[x];
+      if (qssok) sq = qss[y];
+      if (tssok) st = tss[x];
+      if (cq == ct) { cm = cq; idts++; }
+      else if (scores[a] > 0.0f) { cm = '+'; psts++; }
+      else cm = ' ';
       x--; y--;
+    }
+    if (o && n < rowcap) {
+      o[n] = cq; o[rowcap + n] = cm; o[2 * rowcap + n] = ct; o[3 * rowcap + n] = sq; o[4 * rowcap + n] = st;
     }
     n++;
   }
@@ -391,11 +414,12 @@ extern "C" int cth_selftest_dpp_shl(int* host_out64) {
 // The MAP stage for njobs profiles of the chunk starting at column col0
 // (query qp of length lq; ltmax the longest job profile): post [lq][ldsc]
 // floats of scratch, btck [lq][ldsc] bytes (overwritten for the job
-// profiles), zrow njobs x lq doubles of scratch, res one record per job.
+// profiles), zrow njobs x lq doubles of scratch, alnout (optional) njobs x 5
+// x (lq + ltmax) bytes for the alignment rows, res one record per job.
 extern "C" int cth_launch_map(const cth_dpset* q, int qp, int lq, const cth_dpset* db, const cth_mapjob* jobs,
                               int njobs, int ltmax, int64_t col0, int64_t ldsc, const float* scores,
                               const float* mods, float* post, uint8_t* btck, double* zrow, float minpp,
-                              cth_mapres* res, hipStream_t st) {
+                              char* alnout, cth_mapres* res, hipStream_t st) {
   if (njobs <= 0) return 0;
   const size_t lds5 = sizeof(float) * 5 * (size_t)(ltmax > 0 ? ltmax : 1);
   if (lds5 > 160 * 1024) { cth_set_error("map: profile length %d exceeds the LDS budget", ltmax); return -1; }
@@ -408,7 +432,8 @@ extern "C" int cth_launch_map(const cth_dpset* q, int qp, int lq, const cth_dpse
   hipLaunchKernelGGL(k_map_align, dim3(njobs), dim3(64), sizeof(float) * (size_t)(ltmax > 0 ? ltmax : 1), st, *q, qp,
                      *db, jobs, col0, ldsc, post, btck, minpp, res);
   CTH_HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_map_fin, dim3(njobs), dim3(64), 0, st, *q, qp, *db, jobs, col0, ldsc, scores, btck, res);
+  hipLaunchKernelGGL(k_map_fin, dim3(njobs), dim3(64), 0, st, *q, qp, *db, jobs, col0, ldsc, scores, btck, alnout,
+                     lq + ltmax, res);
   CTH_HIP_CHECK(hipGetLastError());
   return 0;
 }

@@ -33,9 +33,9 @@
 extern "C" int cth_launch_score(const cth_dpset* q, int qp, const cth_dpset* db, int64_t col0, int ncols,
                                 int64_t ldsc, const cth_tables* dtabs, int lq, float* scores, float* mods,
                                 hipStream_t st);
-extern "C" int cth_launch_dp(const cth_dpset* q, int qp, const cth_dpset* db, int p0, int np, int ltmax,
-                             int64_t col0, int64_t ldsc, const float* scores, uint8_t* btck, float scorethld,
-                             cth_dpres* res, hipStream_t st);
+extern "C" int cth_launch_dp_list(const cth_dpset* q, int qp, const cth_dpset* db, int p0, const int* plist, int np,
+                                  int ltmax, int64_t col0, int64_t ldsc, const float* scores, uint8_t* btck,
+                                  float scorethld, int finalize, cth_dpres* res, hipStream_t st);
 extern "C" int cth_launch_stats(const cth_dpset* q, int qp, const cth_dpset* db, int p0, int64_t col0, int64_t ldsc,
                                 const float* scores, const int* plist_abs, const int* plist_rel,
                                 const cth_dpres* dpres, int np2, double searchspace, int ssemodel, int* hist,
@@ -43,7 +43,7 @@ extern "C" int cth_launch_stats(const cth_dpset* q, int qp, const cth_dpset* db,
 extern "C" int cth_launch_map(const cth_dpset* q, int qp, int lq, const cth_dpset* db, const cth_mapjob* jobs,
                               int njobs, int ltmax, int64_t col0, int64_t ldsc, const float* scores,
                               const float* mods, float* post, uint8_t* btck, double* zrow, float minpp,
-                              cth_mapres* res, hipStream_t st);
+                              char* alnout, cth_mapres* res, hipStream_t st);
 
 namespace {
 
@@ -115,6 +115,7 @@ extern "C" int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt
     c->has_ev = 1;
   }
   int nh = 0;
+  uint64_t alnused = 0;
   cth_search_stats st{};
   for (int qi = q0; qi < q1; qi++) {
     const int lq = c->q.h_len[qi];
@@ -143,15 +144,18 @@ extern "C" int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt
       if (cth_launch_score(&c->q.d, qi, &db.d, col0, (int)ncols, ldsc, c->dtab, lq, c->scores, c->mods, c->st))
         return -1;
       if (timing) CTH_HIP_CHECK(hipEventRecord(c->ev[1], c->st));
-      if (cth_launch_dp(&c->q.d, qi, &db.d, p0, p1 - p0, ltmax, col0, ldsc, c->scores, c->btck, scorethld,
-                        (cth_dpres*)c->dpout, c->st))
+      // phase 1 in two passes: the DP of every profile without traceback,
+      // then, for the profiles whose maximum exceeds the score threshold (the
+      // only ones FinalizeDPPro traces back, cudp/CuBatchDP_final.cu:144),
+      // the same DP with traceback and the finalisation
+      if (cth_launch_dp_list(&c->q.d, qi, &db.d, p0, nullptr, p1 - p0, ltmax, col0, ldsc, c->scores, nullptr,
+                             scorethld, 0, (cth_dpres*)c->dpout, c->st))
         return -1;
       if (timing) CTH_HIP_CHECK(hipEventRecord(c->ev[2], c->st));
       std::vector<cth_dpres> res(p1 - p0);
       CTH_HIP_CHECK(hipMemcpyAsync(res.data(), c->dpout, sizeof(cth_dpres) * res.size(), hipMemcpyDeviceToHost,
                                    c->st));
       CTH_HIP_CHECK(hipStreamSynchronize(c->st));
-      const double t1 = now_ms();
       if (timing) {
         float a = 0.0f, b = 0.0f;
         CTH_HIP_CHECK(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
@@ -159,6 +163,42 @@ extern "C" int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt
         st.ms_k_score += a;
         st.ms_k_dp += b;
       }
+      std::vector<int> cand, cand_abs;
+      int ltc = 0;
+      for (int k = 0; k < p1 - p0; k++) {
+        res[k].passed = 0; res[k].score = 0.0f; res[k].psts = 0; res[k].beg = 0;
+        if (scorethld < res[k].maxsc) {
+          cand.push_back(k);
+          cand_abs.push_back(p0 + k);
+          ltc = std::max(ltc, db.h_len[p0 + k]);
+        }
+      }
+      if (!cand.empty()) {
+        const int nc = (int)cand.size();
+        const size_t wneed = sizeof(int) * nc + sizeof(cth_dpres) * nc + 256;
+        if (cth_ctx_reserve(c, &c->work, &c->cap_work, wneed)) return -1;
+        int* d_cand = (int*)c->work;
+        cth_dpres* d_res2 = (cth_dpres*)(((uintptr_t)(d_cand + nc) + 15) & ~(uintptr_t)15);
+        CTH_HIP_CHECK(hipMemcpyAsync(d_cand, cand_abs.data(), sizeof(int) * nc, hipMemcpyHostToDevice, c->st));
+        if (timing) CTH_HIP_CHECK(hipEventRecord(c->ev[1], c->st));
+        if (cth_launch_dp_list(&c->q.d, qi, &db.d, p0, d_cand, nc, ltc, col0, ldsc, c->scores, c->btck, scorethld,
+                               1, d_res2, c->st))
+          return -1;
+        if (timing) CTH_HIP_CHECK(hipEventRecord(c->ev[2], c->st));
+        std::vector<cth_dpres> res2(nc);
+        CTH_HIP_CHECK(hipMemcpyAsync(res2.data(), d_res2, sizeof(cth_dpres) * nc, hipMemcpyDeviceToHost, c->st));
+        CTH_HIP_CHECK(hipStreamSynchronize(c->st));
+        for (int k = 0; k < nc; k++) res[cand[k]] = res2[k];
+        // the statistics kernel reads the phase-1 records of the chunk from dpout
+        CTH_HIP_CHECK(hipMemcpyAsync(c->dpout, res.data(), sizeof(cth_dpres) * res.size(), hipMemcpyHostToDevice,
+                                     c->st));
+        if (timing) {
+          float b = 0.0f;
+          CTH_HIP_CHECK(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+          st.ms_k_dp += b;
+        }
+      }
+      const double t1 = now_ms();
       std::vector<int> rel, abs_;
       for (int k = 0; k < p1 - p0; k++)
         if (res[k].passed) { rel.push_back(k); abs_.push_back(p0 + k); }
@@ -194,6 +234,8 @@ extern "C" int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt
       for (int j = 0; j < np2; j++)
         if (!(logethld < (float)sv[(size_t)j * CTH_STATS_NOUT + CTH_ST_LOGE])) sel.push_back(j);
       std::vector<cth_mapres> mres(sel.size());
+      std::vector<char> hrows;  // alignment rows of the jobs, in traceback order
+      int rowcap = 0;
       if (opt->do_map && !sel.empty()) {
         const int nj = (int)sel.size();
         std::vector<cth_mapjob> jobs(nj);
@@ -204,18 +246,26 @@ extern "C" int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt
           ltm = std::max(ltm, db.h_len[jobs[k].profile]);
         }
         if (cth_ctx_reserve(c, (void**)&c->mapbuf, &c->cap_mapbuf, need)) return -1;
-        const size_t wneed = sizeof(double) * (size_t)nj * lq + sizeof(cth_mapjob) * nj + sizeof(cth_mapres) * nj + 512;
+        rowcap = lq + ltm;
+        const size_t alnbytes = opt->alnbuf ? (size_t)nj * 5 * rowcap : 0;
+        const size_t wneed = sizeof(double) * (size_t)nj * lq + sizeof(cth_mapjob) * nj + sizeof(cth_mapres) * nj +
+                             alnbytes + 512;
         if (cth_ctx_reserve(c, &c->work, &c->cap_work, wneed)) return -1;
         double* d_zrow = (double*)c->work;
         cth_mapjob* d_jobs = (cth_mapjob*)(d_zrow + (size_t)nj * lq);
         cth_mapres* d_res = (cth_mapres*)(((uintptr_t)(d_jobs + nj) + 15) & ~(uintptr_t)15);
+        char* d_aln = alnbytes ? (char*)(d_res + nj) : nullptr;
         CTH_HIP_CHECK(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(cth_mapjob) * nj, hipMemcpyHostToDevice, c->st));
         if (timing) CTH_HIP_CHECK(hipEventRecord(c->ev[3], c->st));
         if (cth_launch_map(&c->q.d, qi, lq, &db.d, d_jobs, nj, ltm, col0, ldsc, c->scores, c->mods, c->mapbuf,
-                           c->btck, d_zrow, opt->minpp > 0.0f ? opt->minpp : CTH_MINPP, d_res, c->st))
+                           c->btck, d_zrow, opt->minpp > 0.0f ? opt->minpp : CTH_MINPP, d_aln, d_res, c->st))
           return -1;
         if (timing) CTH_HIP_CHECK(hipEventRecord(c->ev[0], c->st));
         CTH_HIP_CHECK(hipMemcpyAsync(mres.data(), d_res, sizeof(cth_mapres) * nj, hipMemcpyDeviceToHost, c->st));
+        if (alnbytes) {
+          hrows.resize(alnbytes);
+          CTH_HIP_CHECK(hipMemcpyAsync(hrows.data(), d_aln, alnbytes, hipMemcpyDeviceToHost, c->st));
+        }
         CTH_HIP_CHECK(hipStreamSynchronize(c->st));
         if (timing) {
           float a = 0.0f;
@@ -264,6 +314,19 @@ extern "C" int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt
           h.alnlen = mres[k].alnlen;
           h.pstvs_map = mres[k].psts;
           h.mapscore = mres[k].maxsc;
+          if (opt->alnbuf) {
+            // five rows of alnlen letters each, in reading order
+            const size_t n = (size_t)mres[k].alnlen;
+            if (alnused + 5 * n > opt->alncap) {
+              cth_set_error("search: alignment buffer full (%llu bytes)", (unsigned long long)opt->alncap);
+              return -1;
+            }
+            h.alnoff = alnused;
+            const char* src = hrows.data() + k * 5 * (size_t)rowcap;
+            for (int r = 0; r < 5; r++)
+              for (size_t m = 0; m < n; m++) opt->alnbuf[alnused + r * n + m] = src[r * (size_t)rowcap + n - 1 - m];
+            alnused += 5 * n;
+          }
         }
       }
       p0 = p1;

@@ -121,6 +121,8 @@ typedef struct cth_search_opts {
   uint64_t mem_budget;/* bytes for the per-chunk score/traceback buffers (0: 16 GiB) */
   float minpp;        /* MAP posterior threshold (MINPP; 0: the default 0.28) */
   int reserved;
+  char* alnbuf;       /* optional: receives the alignment rows of every hit (see cth_hit.alnoff) */
+  uint64_t alncap;    /* capacity of alnbuf in bytes */
 } cth_search_opts;
 
 /* One reported hit; fields follow the reference's per-profile output record
@@ -136,6 +138,11 @@ typedef struct cth_hit {
   uint32_t mapbeg, mapend;    /* MAP alignment begin/end coordinates */
   int32_t idnts, ngaps, alnlen, pstvs_map;  /* identities, gaps, length, positives of the MAP alignment */
   float mapscore;             /* MAP alignment score */
+  int32_t reserved;
+  uint64_t alnoff;            /* with cth_search_opts.alnbuf: offset of this hit's five rows of alnlen
+                                 letters each (query, middle, target, query SS, target SS; the output
+                                 rows of FinalizeMAPDP_MAP, src/libmycu/cumapdp/CuBatchMAPDP_map_final.cu:
+                                 228-300) */
 } cth_hit;
 
 typedef struct cth_search_stats {

@@ -130,6 +130,8 @@ typedef struct orc_hit {
   uint32_t beg, end, mapbeg, mapend;
   int32_t idnts, ngaps, alnlen, pstvs_map;
   float mapscore;
+  int32_t reserved;
+  uint64_t alnoff;
 } orc_hit;
 int orc_search_query(const orc_tables* t, const orc_pset* q, int qp, const orc_pset* db, int p0, int p1,
                      float eval, int ssemodel, int do_map, uint64_t dbsize, uint64_t ndb, orc_hit* hits, int maxhits,

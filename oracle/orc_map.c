@@ -304,45 +304,57 @@ void orc_map_align(int lq, int lt, const float* post, float logz, float minpp, u
 /* FinalizeMAPDP_MAP traceback (cumapdp/CuBatchMAPDP_map_final.cu:210-336):
  * from the maximum (or from (0,0) when it is not positive) follow the
  * traceback bytes; identities compare residue letters, positives are the
- * other aligned pairs with an original score > 0. */
+ * other aligned pairs with an original score > 0.  aln (optional): five rows
+ * of alncap bytes (query, middle, target, query SS, target SS), filled in
+ * reading order; a profile's SS letters are used when it has them (first
+ * letter nonzero). */
 void orc_map_trace(const orc_pset* q, int qp, const orc_pset* db, int tp, const float* sc, int64_t ldsc, int64_t col,
                    const uint8_t* bt, float maxsc, uint32_t maxxy, orc_mapres* out, char* aln, int alncap) {
   const int lt = db->len[tp];
   const char* qaa = q->aa + q->dist[qp];
   const char* taa = db->aa + db->dist[tp];
+  const char* qss = q->sss + q->dist[qp];
+  const char* tss = db->sss + db->dist[tp];
+  const int qssok = qss[0] != 0, tssok = tss[0] != 0;
   uint32_t xx = (0.0f < maxsc) ? maxxy : 0u;
   int y = (int)(xx & 0xffff), x = (int)(xx >> 16);
   int idts = 0, psts = 0, gaps = 0, n = 0;
-  char* qa = aln;
-  char* mi = aln ? aln + alncap : NULL;
-  char* ta = aln ? aln + 2 * alncap : NULL;
   for (;;) {
     if (x < 0 || y < 0) break;
     const int b = bt[(int64_t)y * lt + x];
     if (b == BT_STOP) break;
-    char cq, cm, ct;
+    char cq, cm, ct, sq = ' ', st = ' ';
     if (b == BT_UP) {
-      gaps++; cq = qaa[y]; ct = '-'; cm = ' '; y--;
+      gaps++; cq = qaa[y]; ct = '-'; cm = ' ';
+      if (qssok) sq = qss[y];
+      y--;
     } else if (b == BT_LEFT) {
-      gaps++; cq = '-'; ct = taa[x]; cm = ' '; x--;
+      gaps++; cq = '-'; ct = taa[x]; cm = ' ';
+      if (tssok) st = tss[x];
+      x--;
     } else {
       cq = qaa[y]; ct = taa[x];
+      if (qssok) sq = qss[y];
+      if (tssok) st = tss[x];
       if (cq == ct) { cm = cq; idts++; }
       else if (sc[(int64_t)y * ldsc + col + x] > 0.0f) { cm = '+'; psts++; }
       else cm = ' ';
       x--; y--;
     }
-    if (aln && n < alncap) { qa[n] = cq; mi[n] = cm; ta[n] = ct; }
+    if (aln && n < alncap) {
+      aln[n] = cq; aln[alncap + n] = cm; aln[2 * alncap + n] = ct;
+      aln[3 * alncap + n] = sq; aln[4 * alncap + n] = st;
+    }
     n++;
   }
   if (aln) {
     /* the walk runs from the end: reverse into reading order */
     const int m = n < alncap ? n : alncap;
-    for (int k = 0; k < m / 2; k++) {
-      char t;
-      t = qa[k]; qa[k] = qa[m - 1 - k]; qa[m - 1 - k] = t;
-      t = mi[k]; mi[k] = mi[m - 1 - k]; mi[m - 1 - k] = t;
-      t = ta[k]; ta[k] = ta[m - 1 - k]; ta[m - 1 - k] = t;
+    for (int r = 0; r < 5; r++) {
+      char* row = aln + (size_t)r * alncap;
+      for (int k = 0; k < m / 2; k++) {
+        const char t = row[k]; row[k] = row[m - 1 - k]; row[m - 1 - k] = t;
+      }
     }
   }
   out->idnts = idts;

@@ -14,6 +14,8 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <stdlib.h>
+
 #include "oracle.h"
 
 #define NAA 20
@@ -246,25 +248,28 @@ void orc_score_matrix(const orc_tables* t, const orc_pset* q, int qp, const orc_
   const int64_t c0 = db->dist[p0];
   const int64_t ncols = (p1 > p0) ? db->dist[p1 - 1] + db->len[p1 - 1] - c0 : 0;
   if (nthreads < 1) nthreads = 1;
-#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
-  for (int p = p0; p < p1; p++) {
-    for (int m = 0; m < db->len[p]; m++) {
-      const int64_t tj = db->dist[p] + m;
-      const int64_t col = tj - c0;
-      for (int r = 0; r < lq; r++) {
-        const int64_t qi = q0 + r;
-        float s = orc_cell_init(q, qi, qp, db, tj, p);
-        const float ssc = orc_cell_ss(t, q, qi, db, tj) + orc_cell_cvs(t, q, qi, db, tj);
-        s = s + ssc;
-        float ms = fmaf(-0.5f, 0.15f, ssc); /* score - CONSTCVSSHIFT*cvswgt */
-        const float d = orc_cell_ddms(t, q->dd + (size_t)qi * DDSTRIDE, db->dd + (size_t)tj * DDSTRIDE);
-        if (d) {
-          s = s + d;
-          ms = ms + fmaf(-0.2f, 0.2f, d); /* score - CONSTDDMSSHIFT*ddmswgt */
-        }
-        scores[(int64_t)r * ncols + col] = s;
-        modscores[(int64_t)r * ncols + col] = ms;
+  /* profile index of every column */
+  int* prof = (int*)malloc(sizeof(int) * (size_t)(ncols > 0 ? ncols : 1));
+  for (int p = p0; p < p1; p++)
+    for (int m = 0; m < db->len[p]; m++) prof[db->dist[p] + m - c0] = p;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
+  for (int64_t col = 0; col < ncols; col++) {
+    const int p = prof[col];
+    const int64_t tj = c0 + col;
+    for (int r = 0; r < lq; r++) {
+      const int64_t qi = q0 + r;
+      float s = orc_cell_init(q, qi, qp, db, tj, p);
+      const float ssc = orc_cell_ss(t, q, qi, db, tj) + orc_cell_cvs(t, q, qi, db, tj);
+      s = s + ssc;
+      float ms = fmaf(-0.5f, 0.15f, ssc); /* score - CONSTCVSSHIFT*cvswgt */
+      const float d = orc_cell_ddms(t, q->dd + (size_t)qi * DDSTRIDE, db->dd + (size_t)tj * DDSTRIDE);
+      if (d) {
+        s = s + d;
+        ms = ms + fmaf(-0.2f, 0.2f, d); /* score - CONSTDDMSSHIFT*ddmswgt */
       }
+      scores[(int64_t)r * ncols + col] = s;
+      modscores[(int64_t)r * ncols + col] = ms;
     }
   }
+  free(prof);
 }

@@ -42,7 +42,10 @@ int orc_search_query(const orc_tables* t, const orc_pset* q, int qp, const orc_p
   orc_hit* all = (orc_hit*)calloc((size_t)(n > 0 ? n : 1), sizeof(orc_hit));
   char* keep = (char*)calloc((size_t)(n > 0 ? n : 1), 1);
   const char* qaa = q->aa + q->dist[qp];
-#pragma omp parallel num_threads(nthreads > 0 ? nthreads : 1)
+  /* few long profiles: one at a time with the threads inside the score
+   * matrix; many: the threads over profiles */
+  const int inner = (n < nthreads) ? nthreads : 1;
+#pragma omp parallel num_threads(inner > 1 ? 1 : (nthreads > 0 ? nthreads : 1))
   {
     float* sc = NULL;
     float* md = NULL;
@@ -60,7 +63,7 @@ int orc_search_query(const orc_tables* t, const orc_pset* q, int qp, const orc_p
         bt = (uint8_t*)malloc(need);
         cap = need;
       }
-      orc_score_matrix(t, q, qp, db, p, p + 1, sc, md, 1);
+      orc_score_matrix(t, q, qp, db, p, p + 1, sc, md, inner);
       float maxsc, score;
       uint32_t maxxy, beg;
       int passed, psts;

@@ -153,11 +153,11 @@ class Oracle:
     def map_profile(self, q, qp, db, p, sc, md, col, logepa, minpp=0.28, want_aln=False):
         """MAP stage of query qp vs profile p on the score/modular-score block
         sc, md ([Lq][ncols], profile p at column col): a MAPRES record (and the
-        query/middle/target alignment strings)."""
+        five alignment rows: query, middle, target, query SS, target SS)."""
         from cother_amd import MAPRES
         res = np.zeros(1, MAPRES)
         cap = int(q.lens()[qp]) + int(db.lens()[p]) + 2
-        aln = ctypes.create_string_buffer(3 * cap) if want_aln else None
+        aln = ctypes.create_string_buffer(5 * cap) if want_aln else None
         s_ = np.ascontiguousarray(sc, np.float32)
         m_ = np.ascontiguousarray(md, np.float32)
         self.L.orc_map_profile(ctypes.cast(q.ptr, ctypes.c_void_p), qp, ctypes.cast(db.ptr, ctypes.c_void_p), p,
@@ -167,7 +167,7 @@ class Oracle:
             return res[0]
         n = int(res[0]["alnlen"])
         raw = aln.raw
-        return res[0], (raw[:n].decode(), raw[cap:cap + n].decode(), raw[2 * cap:2 * cap + n].decode())
+        return res[0], tuple(raw[r * cap:r * cap + n].decode() for r in range(5))
 
     def ddms_tab(self, tab22, qrec, trec):
         tab = np.ascontiguousarray(tab22, np.float32)

@@ -57,3 +57,13 @@ def test_synth_distance_lists():
     # values beyond the count are the missing marker
     for i in range(0, db.npos, 97):
         assert (dd[i, 1 + n[i]:] == 255).all()
+
+
+def test_concat_keeps_profiles():
+    a = cother_amd.synth(2, 0, lmin=5, lmax=20, dist_mode=1, seed=1)
+    b = cother_amd.synth(3, 0, lmin=5, lmax=20, dist_mode=0, seed=2)
+    c = cother_amd.concat([a, b])
+    assert list(c.lens()) == list(a.lens()) + list(b.lens())
+    assert (c.field("trn")[:, a.npos + a.nprof:] == b.field("trn")).all()
+    assert (c.field("dd")[a.npos:] == b.field("dd")).all()
+    assert (c.field("bkg")[a.nprof:] == b.field("bkg")).all()

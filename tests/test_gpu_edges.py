@@ -1,0 +1,60 @@
+"""Edge cases of the whole search on the GPU against the oracle: tiny profiles
+(shorter than the DDMS segment, single positions, empty distance lists),
+query lengths around the 32-row block and 64-row stripe sizes, and long
+profiles (the long-profile workload's shape: Lq = 2000 against templates of
+1000-3000 positions, with a self-hit whose MAP alignment spans 32 stripes)."""
+import numpy as np
+import pytest
+
+import cother_amd
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("profile", "score", "psts", "beg", "end", "mapbeg", "mapend", "idnts", "ngaps", "alnlen", "pstvs_map",
+          "mapscore")
+STATS = ("logE", "logEpA", "lambda_est", "K_est", "minsc", "maxsc", "expected", "H", "lambda_", "K")
+
+
+def _compare(ctx, oracle, q, db, eval_, **kw):
+    ctx.upload_queries(q)
+    ctx.upload_db(db)
+    hits, nh, _ = ctx.search(eval=eval_, **kw)
+    off = 0
+    total = 0
+    for qi in range(q.nprof):
+        got = hits[off:off + nh[qi]]
+        off += nh[qi]
+        want = oracle.search_query(q, qi, db, eval_=eval_)
+        print(f"query {qi} (Lq={q.lens()[qi]}): {len(got)} hits, oracle {len(want)}", flush=True)
+        assert len(got) == len(want)
+        for f in FIELDS:
+            assert (got[f] == want[f]).all(), (qi, f)
+        for f in STATS:
+            np.testing.assert_allclose(got[f], want[f], rtol=2e-6, atol=1e-30, err_msg=f)
+        total += len(got)
+    return total
+
+
+def test_tiny_profiles(ctx, oracle):
+    q = cother_amd.synth(3, 0, lmin=1, lmax=9, dist_mode=0, sort_desc=0, seed=81)
+    db = cother_amd.synth(40, 0, lmin=1, lmax=8, dist_mode=1, sort_desc=0, seed=82)
+    _compare(ctx, oracle, q, db, 1e6)
+
+
+@pytest.mark.parametrize("lq", [7, 31, 32, 33, 63, 64, 65, 97])
+def test_query_lengths_around_block_sizes(ctx, oracle, lq):
+    q = cother_amd.synth(1, 2, lmin=lq, lmax=lq, dist_mode=0, sort_desc=0, seed=90 + lq)
+    db = cother_amd.synth(30, 0, lmin=5, lmax=140, dist_mode=1, sort_desc=1, seed=91 + lq)
+    _compare(ctx, oracle, q, db, 1e4)
+
+
+def test_long_profiles_with_self_hit(ctx, oracle):
+    q = cother_amd.synth(1, 2, lmin=2000, lmax=2000, dist_mode=1, sort_desc=0, seed=95)
+    # template 0 is the query itself; two more long random templates
+    same = cother_amd.synth(1, 2, lmin=2000, lmax=2000, dist_mode=1, sort_desc=0, seed=95)
+    other = cother_amd.synth(2, 0, lmin=1000, lmax=3000, dist_mode=1, sort_desc=0, seed=96)
+    db = cother_amd.concat([same, other])
+    n = _compare(ctx, oracle, q, db, 10.0)
+    assert n >= 1
+    hits, _, _ = ctx.search(eval=10.0)
+    assert hits[0]["profile"] == 0 and hits[0]["alnlen"] > 1900

@@ -53,3 +53,22 @@ def test_map_self_alignments(ctx, oracle):
     for qi in range(3):
         res = _check(ctx, oracle, q, db, qi, 0, 3, np.arange(3, dtype=np.int32), np.full(3, -30.0, np.float32))
         assert res[qi]["alnlen"] > 0.8 * q.lens()[qi], res[qi]
+
+
+def test_alignment_rows_match_oracle(ctx, oracle):
+    # self hits (long alignments) plus chance hits among random profiles
+    q = cother_amd.synth(2, 0, lmin=60, lmax=200, dist_mode=1, sort_desc=0, seed=72)
+    same = cother_amd.synth(2, 0, lmin=60, lmax=200, dist_mode=1, sort_desc=0, seed=72)
+    other = cother_amd.synth(12, 0, lmin=30, lmax=300, dist_mode=1, sort_desc=0, seed=73)
+    db = cother_amd.concat([same, other])
+    ctx.upload_queries(q)
+    ctx.upload_db(db)
+    hits, nh, _, alns = ctx.search(eval=100.0, alignments=True)
+    assert len(hits) >= 2
+    d = db.dists()
+    for h, rows in zip(hits, alns):
+        qi, p = int(h["query"]), int(h["profile"])
+        sc, md = oracle.score_matrix(q, qi, db, 0, db.nprof)
+        res, want = oracle.map_profile(q, qi, db, p, sc, md, int(d[p]), float(h["logEpA"]), want_aln=True)
+        assert int(res["alnlen"]) == int(h["alnlen"]) and rows == want, (qi, p)
+    assert any(int(h["alnlen"]) > 50 for h in hits)

@@ -162,7 +162,7 @@ def test_traceback_path_invariants(oracle, L):
     L.orc_map_align(lq, lt, vp(post), float(logz), 0.28, vp(bt), ctypes.byref(ms), ctypes.byref(mc))
     res = MapRes()
     cap = lq + lt + 2
-    aln = ctypes.create_string_buffer(3 * cap)
+    aln = ctypes.create_string_buffer(5 * cap)
     L.orc_map_trace(qv, 0, dv, 0, vp(sc), lt, 0, vp(bt), ms.value, mc.value, ctypes.byref(res), aln, cap)
     assert res.alnlen >= 40 and res.end == mc.value
     # walk the path and re-add the gains: sum of (exp(P - log Z) - MINPP) on diagonal
@@ -183,5 +183,12 @@ def test_traceback_path_invariants(oracle, L):
     assert tot == pytest.approx(ms.value, abs=1e-4)
     assert res.beg == (((x + 1) << 16) | (y + 1))
     s = aln.raw
-    qa, ta = s[:n].decode(), s[2 * cap:2 * cap + n].decode()
+    qa, mid, ta = s[:n].decode(), s[cap:cap + n].decode(), s[2 * cap:2 * cap + n].decode()
+    qs, ts = s[3 * cap:3 * cap + n].decode(), s[4 * cap:4 * cap + n].decode()
     assert len(qa.replace("-", "")) == (mc.value & 0xffff) - y and len(ta.replace("-", "")) == (mc.value >> 16) - x
+    # middle row: the letter on identities, '+' on positives, blank otherwise and on gaps
+    assert sum(1 for a, m in zip(qa, mid) if m == a) == res.idnts and mid.count("+") == res.psts
+    assert all(m == " " for a, b, m in zip(qa, ta, mid) if a == "-" or b == "-")
+    # SS rows: blank opposite a gap, an SS letter elsewhere
+    assert all((c == " ") == (a == "-") for a, c in zip(qa, qs)) and set(qs) <= set("CEH ")
+    assert all((c == " ") == (b == "-") for b, c in zip(ta, ts)) and set(ts) <= set("CEH ")
