@@ -86,7 +86,8 @@ __device__ __forceinline__ uint32_t mkc(int qb, int qe, int pb, int pe) {
 //
 // Branch-free form of the reference's per-cell update: a segment's begin
 // index is kept on one axis only (the other is that index plus the diagonal
-// offset); "score > 0 and better than the step's best" reduces to
+// offset) and relative to the current 4-cell step, so that the per-cell
+// indices are the literals 1..4; "score > 0 and better than the step's best" reduces to
 // "better than the step's best" because the best starts at 0; and the top-3
 // register is entered only when the step's best can displace its smallest
 // entry (the register stays ordered m0 >= m1 >= m2 >= 0).  The clamp at 0
@@ -96,13 +97,12 @@ __device__ __forceinline__ uint32_t mkc(int qb, int qe, int pb, int pe) {
 // registers.
 #define CTH_DDMS_UPD(K, SK, IDX)                                               \
   {                                                                           \
-    const int ik = (IDX) + (K);                                               \
-    segb = (segsco == 0.0f) ? ik : segb;                                      \
+    segr = (segsco != 0.0f) ? segr : (K);                                     \
     segsco = fmaxf(segsco + (SK), 0.0f);                                      \
-    const bool take = (scom < segsco) & (segb <= ik + 1 - CTH_DSTSEGM);       \
+    const bool take = (scom < segsco) & (segr <= (K) + 1 - CTH_DSTSEGM);      \
     scom = take ? segsco : scom;                                              \
-    bb = take ? segb : bb;                                                    \
-    be = take ? ik : be;                                                      \
+    bb = take ? segr : bb;                                                    \
+    be = take ? (K) : be;                                                     \
   }
 
 // the pair score of list values whose table byte offsets are qb (query,
@@ -120,10 +120,10 @@ __device__ float ddms_cell(const float* __restrict__ dmsT, const int32_t* __rest
   float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f;
   uint32_t c0 = 0xffffffffu, c1 = 0xffffffffu, c2 = 0xffffffffu;
   // upper part: diagonal j pairs query index i with template index j + i;
-  // segb is the query index where the running segment begins
+  // i + segr is the query index where the running segment begins
   for (int j = len2 >> 2; 0 <= j; j -= 2) {
     const uint8_t* tj = t + 1 + j;
-    int segb = 0;
+    int segr = 4;  // relative to the first step's index 0 after the step's "segr -= 4"  // segment begin relative to the step's first index i (i = -4 before the first step)
     float s0 = pair(dmsT, q[1], tj[0]);
     float segsco = s0 < 0.0f ? 0.0f : s0;
     const int iend = min(len1, len2 - j);
@@ -134,19 +134,21 @@ __device__ float ddms_cell(const float* __restrict__ dmsT, const int32_t* __rest
       const float s1 = 0.5f * (s0 + s2);
       float scom = 0.0f;
       int bb = 0, be = 0;
+      segr -= 4;
       CTH_DDMS_UPD(1, s1, i)
       CTH_DDMS_UPD(2, s2, i)
       CTH_DDMS_UPD(3, s3, i)
       CTH_DDMS_UPD(4, s4, i)
-      if (scom != 0.0f && m2 <= scom) register_max(m0, m1, m2, c0, c1, c2, scom, mkc(bb, be, bb + j, be + j));
+      if (scom != 0.0f && m2 <= scom)
+        register_max(m0, m1, m2, c0, c1, c2, scom, mkc(bb + i, be + i, bb + i + j, be + i + j));
       s0 = s4;
     }
   }
   // lower part: diagonal i pairs query index i + j with template index j;
-  // segb is the template index where the running segment begins
+  // j + segr is the template index where the running segment begins
   for (int i = len1 >> 2; 0 < i; i -= 2) {
     const int32_t* qi = q + 1 + i;
-    int segb = 0;
+    int segr = 4;  // relative to the first step's index 0 after the step's "segr -= 4"
     float s0 = pair(dmsT, qi[0], t[1]);
     float segsco = s0 < 0.0f ? 0.0f : s0;
     const int jend = min(len2, len1 - i);
@@ -157,11 +159,13 @@ __device__ float ddms_cell(const float* __restrict__ dmsT, const int32_t* __rest
       const float s1 = 0.5f * (s0 + s2);
       float scom = 0.0f;
       int bb = 0, be = 0;
+      segr -= 4;
       CTH_DDMS_UPD(1, s1, j)
       CTH_DDMS_UPD(2, s2, j)
       CTH_DDMS_UPD(3, s3, j)
       CTH_DDMS_UPD(4, s4, j)
-      if (scom != 0.0f && m2 <= scom) register_max(m0, m1, m2, c0, c1, c2, scom, mkc(bb + i, be + i, bb, be));
+      if (scom != 0.0f && m2 <= scom)
+        register_max(m0, m1, m2, c0, c1, c2, scom, mkc(bb + j + i, be + j + i, bb + j, be + j));
       s0 = s4;
     }
   }
