@@ -123,7 +123,7 @@ __device__ float ddms_cell(const float* __restrict__ dmsT, const int32_t* __rest
   // i + segr is the query index where the running segment begins
   for (int j = len2 >> 2; 0 <= j; j -= 2) {
     const uint8_t* tj = t + 1 + j;
-    int segr = 4;  // relative to the first step's index 0 after the step's "segr -= 4"  // segment begin relative to the step's first index i (i = -4 before the first step)
+    int segr = 4;  // relative to the first step's index 0 after the step's "segr -= 4"
     float s0 = pair(dmsT, q[1], tj[0]);
     float segsco = s0 < 0.0f ? 0.0f : s0;
     const int iend = min(len1, len2 - j);
