@@ -108,6 +108,11 @@ def lib():
     L.cth_map_block.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                 ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_void_p]
     L.cth_selftest_dpp_shl.argtypes = [ctypes.c_void_p]
+    L.cth_read_text_profiles.restype = P
+    L.cth_read_text_profiles.argtypes = [ctypes.c_char_p]
+    L.cth_read_binary_db.restype = P
+    L.cth_read_binary_db.argtypes = [ctypes.c_char_p]
+    L.cth_write_text_profiles.argtypes = [P, ctypes.c_char_p, ctypes.c_int]
     L.cth_bench_copy.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
     L.cth_search.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(SearchOpts),
                              ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(SearchStats)]
@@ -165,6 +170,53 @@ def synth(nprof, len_mode, median=0.0, sigma=0.0, lmin=0, lmax=0, dist_mode=1, s
         threads = min(16, os.cpu_count() or 1)
     return Pset(lib().cth_synth_make(nprof, len_mode, median, sigma, lmin, lmax, dist_mode, sort_desc, seed,
                                      threads))
+
+
+def read_text_profiles(path):
+    """Profiles of a text profile file or makedb text database (reference format)."""
+    p = lib().cth_read_text_profiles(path.encode())
+    if not p:
+        raise RuntimeError(lib().cth_last_error().decode())
+    return Pset(p)
+
+
+def read_binary_db(path):
+    """A binary profile database written by the reference's db2bin."""
+    p = lib().cth_read_binary_db(path.encode())
+    if not p:
+        raise RuntimeError(lib().cth_last_error().decode())
+    return Pset(p)
+
+
+def write_text_profiles(ps, path, scale=10000):
+    """Write a profile set in the reference's text profile format."""
+    _check(lib().cth_write_text_profiles(ps.ptr, path.encode(), scale), "cth_write_text_profiles")
+
+
+def subset_list(ps, idx):
+    """A profile set of the profiles idx of ps, in that order."""
+    idx = [int(i) for i in idx]
+    lens = ps.lens()[idx].astype(np.int32)
+    out = Pset(lib().cth_pset_alloc(len(lens), lens.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+    d, ln = ps.dists(), ps.lens()
+    od = out.dists()
+    for k, p in enumerate(idx):
+        a, b = int(d[p]), int(d[p] + ln[p])
+        o = int(od[k])
+        for name in ("trg", "ss", "cv"):
+            out.field(name)[:, o:o + (b - a)] = ps.field(name)[:, a:b]
+        for name in ("cvprior", "cvnorm2", "aa", "sss"):
+            out.field(name)[o:o + (b - a)] = ps.field(name)[a:b]
+        out.field("dd")[o:o + (b - a)] = ps.field("dd")[a:b]
+        out.field("trn")[:, o + k:o + k + (b - a) + 1] = ps.field("trn")[:, a + p:b + p + 1]
+        out.field("bkg")[k] = ps.field("bkg")[p]
+        out.field("eno")[k] = ps.field("eno")[p]
+    return out
+
+
+def subset(ps, p0, p1):
+    """A profile set of the profiles [p0,p1) of ps."""
+    return subset_list(ps, range(p0, p1))
 
 
 def concat(sets):

@@ -2,7 +2,8 @@
 //
 // The field set and value conventions follow the reference's packed profile
 // model (cupro/PM2DVectorFields.h:53-87) as filled by its text/binary readers
-// (cupro/IOProfileModelTxt.cu:296-1030): target frequencies and background
+// (cupro/IOProfileModelTxt.cu:296-1030; the binary database sections of
+// libpro/srcpro/PMProfileModel.cpp:1235-1817): target frequencies and background
 // probabilities are k/10000, transitions are stored as log values (x0.4 for
 // IM, II, DM, DD; -32768 for a zero probability), one extra "beginning"
 // transition vector precedes the positions of every profile, context vectors
@@ -31,7 +32,8 @@ typedef struct cth_pset {
   float*   eno;       // [nprof] effective number of observations
   float*   bkg;       // [nprof*20] background probabilities (profile-major)
   char*    aa;        // [npos] residues
-  char*    sss;       // [npos] SS state letter (C/E/H)
+  char*    sss;       // [npos] SS state letter as the device sees it (' ', 'e', 'h' for C, E, H;
+                      // liblib/alpha.cpp:14); 0 throughout a profile without SS predictions
   float*   trg;       // [20*npos] target frequencies, field-major
   float*   trn;       // [7*(npos+nprof)] log transitions, field-major; profile p vector k at dist[p]+p+k
   float*   ss;        // [3*npos] SS state probabilities, field-major

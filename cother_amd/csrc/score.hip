@@ -127,9 +127,12 @@ __device__ float ddms_cell(const float* __restrict__ dmsT, const int32_t* __rest
     float s0 = pair(dmsT, q[1], tj[0]);
     float segsco = s0 < 0.0f ? 0.0f : s0;
     const int iend = min(len1, len2 - j);
+    float s2 = pair(dmsT, q[1 + 2], tj[2]);
+    float s4 = pair(dmsT, q[1 + 4], tj[4]);
     for (int i = 0; i < iend; i += 4) {
-      const float s2 = pair(dmsT, q[1 + i + 2], tj[i + 2]);
-      const float s4 = pair(dmsT, q[1 + i + 4], tj[i + 4]);
+      // the next step's pair scores, loaded one step ahead (lists are padded)
+      const float n2 = pair(dmsT, q[1 + i + 6], tj[i + 6]);
+      const float n4 = pair(dmsT, q[1 + i + 8], tj[i + 8]);
       const float s3 = 0.5f * (s2 + s4);
       const float s1 = 0.5f * (s0 + s2);
       float scom = 0.0f;
@@ -142,6 +145,8 @@ __device__ float ddms_cell(const float* __restrict__ dmsT, const int32_t* __rest
       if (scom != 0.0f && m2 <= scom)
         register_max(m0, m1, m2, c0, c1, c2, scom, mkc(bb + i, be + i, bb + i + j, be + i + j));
       s0 = s4;
+      s2 = n2;
+      s4 = n4;
     }
   }
   // lower part: diagonal i pairs query index i + j with template index j;
@@ -152,9 +157,11 @@ __device__ float ddms_cell(const float* __restrict__ dmsT, const int32_t* __rest
     float s0 = pair(dmsT, qi[0], t[1]);
     float segsco = s0 < 0.0f ? 0.0f : s0;
     const int jend = min(len2, len1 - i);
+    float s2 = pair(dmsT, qi[2], t[1 + 2]);
+    float s4 = pair(dmsT, qi[4], t[1 + 4]);
     for (int j = 0; j < jend; j += 4) {
-      const float s2 = pair(dmsT, qi[j + 2], t[1 + j + 2]);
-      const float s4 = pair(dmsT, qi[j + 4], t[1 + j + 4]);
+      const float n2 = pair(dmsT, qi[j + 6], t[1 + j + 6]);
+      const float n4 = pair(dmsT, qi[j + 8], t[1 + j + 8]);
       const float s3 = 0.5f * (s2 + s4);
       const float s1 = 0.5f * (s0 + s2);
       float scom = 0.0f;
@@ -167,6 +174,8 @@ __device__ float ddms_cell(const float* __restrict__ dmsT, const int32_t* __rest
       if (scom != 0.0f && m2 <= scom)
         register_max(m0, m1, m2, c0, c1, c2, scom, mkc(bb + j + i, be + j + i, bb + j, be + j));
       s0 = s4;
+      s2 = n2;
+      s4 = n4;
     }
   }
   float s = 0.0f;
@@ -178,7 +187,7 @@ __device__ float ddms_cell(const float* __restrict__ dmsT, const int32_t* __rest
 }
 #undef CTH_DDMS_UPD
 
-__global__ __launch_bounds__(256) void k_score(cth_dpset q, int qp, cth_dpset db, int64_t col0, int ncols,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_score(cth_dpset q, int qp, cth_dpset db, int64_t col0, int ncols,
                                                int64_t ldsc, const cth_tables* __restrict__ tabs,
                                                float* __restrict__ scores, float* __restrict__ mods) {
   __shared__ ScoreLds L;

@@ -43,7 +43,7 @@ const float kRobinson[CTH_NAA] = {
     0.073770f, 0.021990f, 0.051420f, 0.090190f, 0.057440f, 0.022430f, 0.038560f,
     0.052030f, 0.071200f, 0.058410f, 0.013300f, 0.032160f, 0.064410f};
 const char kAA[] = "ARNDCQEGHILKMFPSTWYV";  // liblib/alpha.h:31
-const char kSS[] = "CEH";                   // liblib/alpha.h:44
+const char kSS[] = " eh";                   // SS state letters as the device sees them (liblib/alpha.cpp:14)
 
 // adddist options (env/options.txt DSTSEGM/DSTNPOS/DSTSEP6/DSTGAPL/DSTFREE)
 const int kDstSegm = 6, kDstNpos = 12, kDstFree = 3;

@@ -48,6 +48,22 @@ void cth_pset_free(cth_pset* ps);
 cth_pset* cth_synth_make(int nprof, int len_mode, double median, double sigma, int lmin, int lmax,
                          int dist_mode, int sort_desc, uint64_t seed, int nthreads);
 
+/* ---- profile and database files ----------------------------------------------
+ * The reference's formats, read into the conventions above.  Text profiles
+ * ("COTHER profile v2.5": TextWriteProfileCondensed / TextReadProfile,
+ * src/libpro/srcpro/PMProfileModel.cpp:1856-2080, 3527-4620), single files or
+ * the makedb text database (.tprd, src/libpro/srcpro/Db.cpp:590-670); the
+ * binary database written by db2bin ("COTHER profile bin database v2.5",
+ * layout v2.2: Db::MakeBin_v2_2, Db.cpp:337-378, and
+ * CalculateAndWriteAddressTable_v2_2 / BinaryWriteProfile_v2_2,
+ * PMProfileModel.cpp:1235-1817), which replaces the reference's reader of
+ * that file (src/libmycu/cupro/TdDataReader.cpp:385-800).  NULL on error. */
+cth_pset* cth_read_text_profiles(const char* path);
+cth_pset* cth_read_binary_db(const char* path);
+/* Write profiles in the text profile format (TextWriteProfileCondensed);
+ * values are scaled integers (scale 10000 in the reference's files). */
+int cth_write_text_profiles(const cth_pset* ps, const char* path, int scale);
+
 /* ---- context ------------------------------------------------------------------ */
 typedef struct cth_ctx cth_ctx;
 
