@@ -35,6 +35,8 @@ class CthPset(ctypes.Structure):
         ("cvprior", ctypes.POINTER(ctypes.c_float)),
         ("cvnorm2", ctypes.POINTER(ctypes.c_float)),
         ("dd", ctypes.POINTER(ctypes.c_uint8)),
+        ("name", ctypes.POINTER(ctypes.c_char_p)),
+        ("desc", ctypes.POINTER(ctypes.c_char_p)),
     ]
 
 
@@ -116,6 +118,26 @@ def lib():
     L.cth_bench_copy.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
     L.cth_search.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(SearchOpts),
                              ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(SearchStats)]
+    L.cth_pset_name.restype = ctypes.c_char_p
+    L.cth_pset_name.argtypes = [P, ctypes.c_int]
+    L.cth_pset_desc.restype = ctypes.c_char_p
+    L.cth_pset_desc.argtypes = [P, ctypes.c_int]
+    L.cth_pset_set_text.argtypes = [P, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p]
+    L.cth_search_space.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
+                                   ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]
+    L.cth_search_space.restype = None
+    L.cth_outinfo_init.argtypes = [ctypes.POINTER(OutInfo)]
+    L.cth_outinfo_init.restype = None
+    L.cth_format_hit_plain.argtypes = [P, ctypes.c_int, P, ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(OutInfo),
+                                       ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p)]
+    L.cth_format_results_plain.argtypes = [P, ctypes.c_int, P, ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p,
+                                           ctypes.c_uint64, ctypes.POINTER(OutInfo), ctypes.POINTER(ctypes.c_void_p),
+                                           ctypes.POINTER(ctypes.c_uint64)]
+    L.cth_write_results_plain.argtypes = [P, ctypes.c_int, P, ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p,
+                                          ctypes.c_uint64, ctypes.POINTER(OutInfo), ctypes.c_char_p, ctypes.c_char_p,
+                                          ctypes.c_int]
+    L.cth_free_text.argtypes = [ctypes.c_void_p]
+    L.cth_free_text.restype = None
     _lib = L
     return L
 
@@ -211,6 +233,7 @@ def subset_list(ps, idx):
         out.field("trn")[:, o + k:o + k + (b - a) + 1] = ps.field("trn")[:, a + p:b + p + 1]
         out.field("bkg")[k] = ps.field("bkg")[p]
         out.field("eno")[k] = ps.field("eno")[p]
+        set_text(out, k, pset_name(ps, p), pset_desc(ps, p))
     return out
 
 
@@ -229,8 +252,112 @@ def concat(sets):
         out.field(name)[...] = np.concatenate([s_.field(name) for s_ in sets])
     for name in ("bkg", "dd"):
         out.field(name)[...] = np.concatenate([s_.field(name) for s_ in sets], axis=0)
+    k = 0
+    for s_ in sets:
+        for p in range(s_.nprof):
+            set_text(out, k, pset_name(s_, p), pset_desc(s_, p))
+            k += 1
     return out
 
+
+class OutInfo(ctypes.Structure):
+    """Mirror of cth_outinfo (include/cother_hip.h)."""
+    _fields_ = [("qname", ctypes.c_char_p), ("qdesc", ctypes.c_char_p), ("qrysernr", ctypes.c_int32),
+                ("logevthld", ctypes.c_float), ("dbnames", ctypes.POINTER(ctypes.c_char_p)),
+                ("ndbnames", ctypes.c_int32), ("dbsize", ctypes.c_uint64), ("ndbentries", ctypes.c_uint64),
+                ("nohits", ctypes.c_int32), ("noalns", ctypes.c_int32), ("dsclen", ctypes.c_int32),
+                ("dscwidth", ctypes.c_int32), ("alnwidth", ctypes.c_int32), ("show", ctypes.c_int32),
+                ("ssswgt", ctypes.c_float), ("ref_lambda", ctypes.c_float), ("ref_k", ctypes.c_float),
+                ("gap_lambda", ctypes.c_float), ("gap_k", ctypes.c_float), ("progname", ctypes.c_char_p),
+                ("progversion", ctypes.c_char_p)]
+
+
+def outinfo(**kw):
+    """A cth_outinfo with the reference's defaults (cth_outinfo_init) and the given fields;
+    dbnames is a list of str, qname/qdesc str."""
+    o = OutInfo()
+    lib().cth_outinfo_init(ctypes.byref(o))
+    keep = []
+    for k, v in kw.items():
+        if k == "dbnames":
+            arr = (ctypes.c_char_p * max(1, len(v)))(*[s.encode() for s in v])
+            keep.append(arr)
+            o.dbnames = ctypes.cast(arr, ctypes.POINTER(ctypes.c_char_p))
+            o.ndbnames = len(v)
+        elif k in ("qname", "qdesc", "progname", "progversion"):
+            b = v.encode() if v is not None else None
+            keep.append(b)
+            setattr(o, k, b)
+        else:
+            setattr(o, k, v)
+    o._keep = keep
+    return o
+
+
+def set_text(ps, k, name, desc):
+    """Set the name and description of profile k."""
+    _check(lib().cth_pset_set_text(ps.ptr, k, None if name is None else name.encode(),
+                                   None if desc is None else desc.encode()), "cth_pset_set_text")
+
+
+def pset_name(ps, k):
+    v = lib().cth_pset_name(ps.ptr, k)
+    return None if v is None else v.decode()
+
+
+def pset_desc(ps, k):
+    v = lib().cth_pset_desc(ps.ptr, k)
+    return None if v is None else v.decode()
+
+
+def search_space(qlen, dbsize, ndb):
+    """(effective search space, length adjustment) of a query of qlen positions."""
+    ss, dl = ctypes.c_uint64(), ctypes.c_uint64()
+    lib().cth_search_space(qlen, dbsize, ndb, ctypes.byref(ss), ctypes.byref(dl))
+    return ss.value, dl.value
+
+
+def _take_text(p):
+    try:
+        return ctypes.string_at(p)
+    finally:
+        lib().cth_free_text(p)
+
+
+def format_hit_plain(qs, qidx, db, hit, rows, **info):
+    """(annotation, alignment record) bytes of one hit; rows: the five alignment rows."""
+    o = outinfo(**info)
+    h = np.array(hit, HIT).reshape(1)
+    buf = b"".join(r if isinstance(r, bytes) else r.encode() for r in rows)
+    a, r = ctypes.c_void_p(), ctypes.c_void_p()
+    _check(lib().cth_format_hit_plain(qs.ptr, qidx, db.ptr, h.ctypes.data, buf, ctypes.byref(o), ctypes.byref(a),
+                                      ctypes.byref(r)), "cth_format_hit_plain")
+    return _take_text(a), _take_text(r)
+
+
+def format_results_plain(qs, qidx, db, hits, alnbuf, **info):
+    """The reference's plain-text results file for query qidx, as bytes."""
+    o = outinfo(**info)
+    h = np.ascontiguousarray(hits, HIT)
+    t, n = ctypes.c_void_p(), ctypes.c_uint64()
+    _check(lib().cth_format_results_plain(qs.ptr, qidx, db.ptr, h.ctypes.data, len(h), alnbuf, len(alnbuf),
+                                          ctypes.byref(o), ctypes.byref(t), ctypes.byref(n)),
+           "cth_format_results_plain")
+    try:
+        return ctypes.string_at(t, n.value)
+    finally:
+        lib().cth_free_text(t)
+
+
+def write_results_plain(qs, qidx, db, hits, alnbuf, outdir, **info):
+    """Write the results file of query qidx into outdir; returns its path."""
+    o = outinfo(**info)
+    h = np.ascontiguousarray(hits, HIT)
+    path = ctypes.create_string_buffer(4096)
+    _check(lib().cth_write_results_plain(qs.ptr, qidx, db.ptr, h.ctypes.data, len(h), alnbuf, len(alnbuf),
+                                         ctypes.byref(o), outdir.encode(), path, len(path)),
+           "cth_write_results_plain")
+    return path.value.decode()
 
 class Context:
     """A device context (cth_ctx): tables, uploaded queries, resident DB."""

@@ -41,6 +41,8 @@ typedef struct cth_pset {
   float*   cvprior;   // [npos] log prior of the context vector
   float*   cvnorm2;   // [npos] squared norm of the context vector
   uint8_t* dd;        // [npos*128] distance lists: n, d1..dn, 255 padding
+  char**   name;      // [nprof] profile names (the FILE field of text profiles), NULL entries allowed
+  char**   desc;      // [nprof] profile descriptions (DESC field, database description section)
 } cth_pset;
 
 #ifdef __cplusplus

@@ -31,6 +31,9 @@
 
 extern "C" cth_pset* cth_pset_alloc(int nprof, const int32_t* lens);
 extern "C" void cth_pset_free(cth_pset* ps);
+extern "C" int cth_pset_set_text(cth_pset* ps, int k, const char* name, const char* desc);
+extern "C" const char* cth_pset_name(const cth_pset* ps, int k);
+extern "C" const char* cth_pset_desc(const cth_pset* ps, int k);
 void cth_set_error(const char* fmt, ...);
 
 namespace {
@@ -244,6 +247,7 @@ cth_pset* to_pset(const std::vector<TextProfile>& ps) {
   for (int p = 0; p < o->nprof; p++) {
     const TextProfile& P = ps[p];
     const int64_t d0 = o->dist[p];
+    if (cth_pset_set_text(o, p, P.file.c_str(), P.desc.c_str())) { cth_pset_free(o); return nullptr; }
     o->eno[p] = P.effnos;
     for (int r = 0; r < CTH_NAA; r++) o->bkg[p * CTH_NAA + r] = P.bkg[r];
     for (int t9 = 0, t = 0; t9 < kNStates; t9++) {
@@ -301,7 +305,7 @@ extern "C" cth_pset* cth_read_text_profiles(const char* path) {
 }
 
 // Write profiles in the text format of TextWriteProfileCondensed
-// (PMProfileModel.cpp:1856-2080); names may be NULL (then "p<k>").
+// (PMProfileModel.cpp:1856-2080); a NULL name or description is written empty.
 extern "C" int cth_write_text_profiles(const cth_pset* ps, const char* path, int scale) {
   if (scale < 1) { cth_set_error("write_text: invalid scale"); return -1; }
   FILE* fp = fopen(path, "w");
@@ -311,8 +315,10 @@ extern "C" int cth_write_text_profiles(const cth_pset* ps, const char* path, int
     const int L = ps->len[p];
     const int64_t d0 = ps->dist[p];
     fprintf(fp, "COTHER profile v2.5\n");
-    fprintf(fp, "DESC: synthetic profile %d\n", p);
-    fprintf(fp, "FILE: p%d\n", p);
+    const char* desc = cth_pset_desc(ps, p);
+    const char* name = cth_pset_name(ps, p);
+    fprintf(fp, "DESC: %s\n", desc ? desc : "");
+    fprintf(fp, "FILE: %s\n", name ? name : "");
     fprintf(fp, "CMD: \n");
     fprintf(fp, "LEN: %d\n", L);
     fprintf(fp, "NOS: %d\n", 1);
@@ -443,6 +449,19 @@ extern "C" cth_pset* cth_read_binary_db(const char* path) {
     for (int k = 0; k < rec[0]; k++) {
       const float d = f32(sDst + 1 + k, i);
       rec[1 + k] = (uint8_t)(d < 0.f ? 0 : (d > 255.f ? 255 : (int)d));
+    }
+  }
+  // descriptions: NUL-terminated strings from addr[sDesc] on, profile k ending
+  // (after its NUL) at the file address held in entry k of the end-address section
+  if (need(sDescEnd, 8 * nprof)) {
+    uint64_t beg = addr[sDesc];
+    for (size_t p = 0; p < nprof; p++) {
+      uint64_t end;
+      memcpy(&end, buf.data() + addr[sDescEnd] + 8 * p, 8);
+      if (end < beg || end > buf.size()) { cth_pset_free(o); cth_set_error("%s: bad description address", path); return nullptr; }
+      const std::string d(buf.data() + beg, strnlen(buf.data() + beg, end - beg));
+      if (cth_pset_set_text(o, (int)p, nullptr, d.c_str())) { cth_pset_free(o); return nullptr; }
+      beg = end;
     }
   }
   // a profile without SS predictions has blank SS states and zero probabilities

@@ -62,8 +62,9 @@ void score_threshold(float eval, uint64_t nqyposs, uint64_t dbsize, float* score
 
 // AbstractScoreMatrix::ComputeLengthAdjustment (libpro/srcsco/AbstractScoreMatrix.cpp:303-395),
 // Gertz's iteration for the edge-effect length l with l = a/lambda ln(K(m-l)(n-Nl)) + b
-uint64_t length_adjustment(uint64_t m, uint64_t n, uint64_t N) {
+uint64_t length_adjustment(uint64_t m, uint64_t n, uint64_t N, uint64_t* deltalen = nullptr) {
   const float lambda = CTH_GAPPED_LAMBDA, K = CTH_GAPPED_K, alpha = CTH_GAPPED_ALPHA, beta = CTH_GAPPED_BETA;
+  if (deltalen) *deltalen = 0;
   if (N < 1) return m * n;
   const float logK = logf(K), alratio = alpha / lambda, max_mn = (float)(m > n ? m : n);
   float length = 0.0f, min_length = 0.0f, max_length = 0.0f, nxt = 0.0f, sspace = 0.0f;
@@ -88,9 +89,13 @@ uint64_t length_adjustment(uint64_t m, uint64_t n, uint64_t N) {
     if (length <= max_length) {
       const float space2 = (m - length) * (n - N * length);
       nxt = alratio * (logK + logf(space2)) + beta;
-      if (length <= nxt) sspace = space2;
+      if (length <= nxt) {
+        min_length = length;
+        sspace = space2;
+      }
     }
   }
+  if (deltalen) *deltalen = (uint64_t)min_length;
   return (uint64_t)sspace;
 }
 
@@ -99,6 +104,13 @@ double now_ms() {
 }
 
 }  // namespace
+
+extern "C" void cth_search_space(uint64_t qlen, uint64_t dbsize, uint64_t ndb, uint64_t* sspace, uint64_t* deltalen) {
+  uint64_t dl = 0;
+  const uint64_t ss = length_adjustment(qlen, dbsize, ndb, &dl);
+  if (sspace) *sspace = ss;
+  if (deltalen) *deltalen = dl;
+}
 
 extern "C" int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt, cth_hit* hits, int maxhits,
                           int* nhits, cth_search_stats* stats) {

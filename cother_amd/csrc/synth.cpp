@@ -342,7 +342,31 @@ void cth_pset_free(cth_pset* ps) {
   free(ps->len); free(ps->dist); free(ps->eno); free(ps->bkg); free(ps->aa); free(ps->sss);
   free(ps->trg); free(ps->trn); free(ps->ss); free(ps->cv); free(ps->cvprior); free(ps->cvnorm2);
   free(ps->dd);
+  for (int p = 0; p < ps->nprof; p++) {
+    if (ps->name) free(ps->name[p]);
+    if (ps->desc) free(ps->desc[p]);
+  }
+  free(ps->name); free(ps->desc);
   free(ps);
+}
+
+// Name and description of profile k (NULL when unset).
+const char* cth_pset_name(const cth_pset* ps, int k) {
+  return ps && ps->name && 0 <= k && k < ps->nprof ? ps->name[k] : nullptr;
+}
+const char* cth_pset_desc(const cth_pset* ps, int k) {
+  return ps && ps->desc && 0 <= k && k < ps->nprof ? ps->desc[k] : nullptr;
+}
+
+// Set (copy) the name and description of profile k; a NULL argument clears it.
+int cth_pset_set_text(cth_pset* ps, int k, const char* name, const char* desc) {
+  if (!ps || k < 0 || k >= ps->nprof || !ps->name || !ps->desc) return -1;
+  char* n = name ? strdup(name) : nullptr;
+  char* d = desc ? strdup(desc) : nullptr;
+  if ((name && !n) || (desc && !d)) { free(n); free(d); return -1; }
+  free(ps->name[k]); ps->name[k] = n;
+  free(ps->desc[k]); ps->desc[k] = d;
+  return 0;
 }
 
 // Allocate an empty profile set with the given lengths.
@@ -367,8 +391,10 @@ cth_pset* cth_pset_alloc(int nprof, const int32_t* lens) {
   ps->cvprior = (float*)calloc(n1, sizeof(float));
   ps->cvnorm2 = (float*)calloc(n1, sizeof(float));
   ps->dd = (uint8_t*)malloc(n1 * CTH_DDSTRIDE);
+  ps->name = (char**)calloc(nprof ? nprof : 1, sizeof(char*));
+  ps->desc = (char**)calloc(nprof ? nprof : 1, sizeof(char*));
   if (!ps->eno || !ps->bkg || !ps->aa || !ps->sss || !ps->trg || !ps->trn || !ps->ss || !ps->cv ||
-      !ps->cvprior || !ps->cvnorm2 || !ps->dd) {
+      !ps->cvprior || !ps->cvnorm2 || !ps->dd || !ps->name || !ps->desc) {
     cth_pset_free(ps);
     return nullptr;
   }
@@ -398,6 +424,13 @@ cth_pset* cth_synth_make(int nprof, int len_mode, double median, double sigma, i
   if (nthreads < 1) nthreads = 1;
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
   for (int p = 0; p < nprof; p++) gen_profile(c, order[p], sl[p], ps, p);
+  for (int p = 0; p < nprof; p++) {
+    char name[64], desc[160];
+    snprintf(name, sizeof name, "synth%d", order[p]);
+    snprintf(desc, sizeof desc, "synthetic profile %d, length %d, seed %llu", order[p], sl[p],
+             (unsigned long long)seed);
+    if (cth_pset_set_text(ps, p, name, desc)) { cth_pset_free(ps); return nullptr; }
+  }
   return ps;
 }
 

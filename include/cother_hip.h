@@ -38,6 +38,15 @@ typedef struct cth_pset cth_pset;
 cth_pset* cth_pset_alloc(int nprof, const int32_t* lens);
 void cth_pset_free(cth_pset* ps);
 
+/* Name and description of profile k (NULL when unset): the FILE and DESC
+ * fields of a text profile (PMProfileModel.cpp:1893-1894, read at :2377-2384),
+ * the description section of the binary database (PMProfileModel.cpp:
+ * 1763-1815); the reference keeps them as PMProfileModel::GetName /
+ * GetDescription.  cth_pset_set_text copies both (NULL clears). */
+const char* cth_pset_name(const cth_pset* ps, int k);
+const char* cth_pset_desc(const cth_pset* ps, int k);
+int cth_pset_set_text(cth_pset* ps, int k, const char* name, const char* desc);
+
 /* Seeded synthetic profile set in the reference's value conventions (no
  * reference counterpart: the reference has no generator; values follow the
  * text profile round trip of src/libpro/srcpro/PMProfileModel.cpp:1856-2082
@@ -180,6 +189,57 @@ typedef struct cth_search_stats {
  * NULL) the cells processed and stage times. */
 int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt, cth_hit* hits, int maxhits, int* nhits,
                cth_search_stats* stats);
+
+/* Length adjustment and effective search space of a query of qlen positions
+ * against dbsize positions in ndb profiles with the gapped parameters
+ * (AbstractScoreMatrix::ComputeLengthAdjustment, src/libpro/srcsco/
+ * AbstractScoreMatrix.cpp:246-395: GetSearchSpace, GetDeltaLength). */
+void cth_search_space(uint64_t qlen, uint64_t dbsize, uint64_t ndb, uint64_t* sspace, uint64_t* deltalen);
+
+/* ---- results file ----------------------------------------------------------
+ * The reference's plain-text output for one query: the per-hit records of
+ * CuBatchProcessingFinalizer::CompressResultsPlain (src/libmycu/cupro/
+ * CuBatchProcessingFinalizerPlain.cpp:36-403) assembled as
+ * AlnWriter::WriteResultsPlain writes them (src/libmycu/cualn/
+ * AlnWriterPlain.cpp:19-398, AlnWriter.cpp:250-376). */
+typedef struct cth_outinfo {
+  const char* qname;          /* query name for the file name (NULL: the query profile's name) */
+  const char* qdesc;          /* query description (NULL: the query profile's description) */
+  int32_t qrysernr;           /* query serial number (file name suffix __<n>.out) */
+  float logevthld;            /* log of the e-value threshold (log EVAL) */
+  const char* const* dbnames; /* database names as given on the command line */
+  int32_t ndbnames;
+  uint64_t dbsize;            /* database positions (also the length adjustment's n) */
+  uint64_t ndbentries;        /* database profiles (the length adjustment's N) */
+  int32_t nohits, noalns;     /* options NOHITS, NOALNS */
+  int32_t dsclen, dscwidth;   /* options DSCLEN, DSCWIDTH */
+  int32_t alnwidth, show;     /* options ALNWIDTH, SHOW */
+  float ssswgt;               /* option SSSWGT: SS rows are printed when > 0 */
+  float ref_lambda, ref_k;    /* ungapped reference values ([UNGAPPED] of cother.par) */
+  float gap_lambda, gap_k;    /* gapped reference values */
+  const char* progname;       /* first line: program name and version */
+  const char* progversion;
+} cth_outinfo;
+
+/* Defaults of the reference: options.txt values, cother.par reference
+ * values, "cother 2.03.03". */
+void cth_outinfo_init(cth_outinfo* o);
+/* The hit-list annotation and the alignment record of one hit whose five
+ * rows (query, middle, target, query SS, target SS; alnlen letters each) are
+ * at rows.  Both strings are allocated; release them with cth_free_text. */
+int cth_format_hit_plain(const cth_pset* qs, int qidx, const cth_pset* db, const cth_hit* hit, const char* rows,
+                         const cth_outinfo* o, char** annot, char** record);
+/* The whole file for query qidx from the hits of a cth_search call with
+ * alignment rows (hits of other queries are skipped; reported are those with
+ * logE <= logevthld and an alignment, by ascending logE). */
+int cth_format_results_plain(const cth_pset* qs, int qidx, const cth_pset* db, const cth_hit* hits, int nhits,
+                             const char* alnbuf, uint64_t alncap, const cth_outinfo* o, char** text, uint64_t* len);
+/* As cth_format_results_plain, written to <outdir>/<query name>__<qrysernr>.out
+ * (AlnWriter::GetOutputFilename); the path is copied to path. */
+int cth_write_results_plain(const cth_pset* qs, int qidx, const cth_pset* db, const cth_hit* hits, int nhits,
+                            const char* alnbuf, uint64_t alncap, const cth_outinfo* o, const char* outdir, char* path,
+                            int pathcap);
+void cth_free_text(char* p);
 
 /* Measurement aid (no reference counterpart): a plain device-to-device copy
  * kernel over `bytes` (multiple of 16), launched `reps` times; *ms receives

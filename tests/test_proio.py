@@ -61,6 +61,9 @@ def test_text_round_trip(tmp_path, sets):
         cother_amd.write_text_profiles(ps, path)
         back = cother_amd.read_text_profiles(path)
         _same(ps, back, skip=("ss",))
+        for p in range(ps.nprof):
+            assert cother_amd.pset_name(back, p) == cother_amd.pset_name(ps, p)
+            assert cother_amd.pset_desc(back, p) == cother_amd.pset_desc(ps, p)
         # the reference writes SS probabilities with a floor: a value may drop by 1/scale
         assert (back.field("ss").view(np.int32) == _ss_written(ps.field("ss")).view(np.int32)).all()
 
@@ -90,6 +93,10 @@ def test_reference_tools_agree(tmp_path, sets):
     order = _match(ps, text)
     src = cother_amd.subset_list(ps, order)
     _same(src, text, skip=("ss",))
+    # descriptions travel through makedb (text database) and db2bin (description section)
+    for p in range(src.nprof):
+        assert cother_amd.pset_desc(text, p) == cother_amd.pset_desc(src, p)
+        assert cother_amd.pset_desc(binary, p) == cother_amd.pset_desc(src, p)
     assert (text.field("ss").view(np.int32) == _ss_written(_ss_written(src.field("ss"))).view(np.int32)).all()
 
 
