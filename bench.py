@@ -109,6 +109,55 @@ def cpu_baseline(q, db, target_s, threads):
                       f"{cells:.3g} cells in {dt:.1f} s, full pipeline"}
 
 
+def serve_phase(ctx, qs, db, args, seq_qps):
+    """Requests arriving as a seeded Poisson process at serve_load x the sequential query
+    rate, run by the serving loop (slots overlap on the device, admission by buffer
+    footprint); latency is submit -> done, throughput is cells over first submit -> last done."""
+    import threading
+    import cother_amd
+    n = args.serve_requests
+    ql = qs.lens()
+    order = [(args.warmup + s) % qs.nprof for s in range(n)]
+    rate = seq_qps * args.serve_load
+    rng = np.random.default_rng(args.seed + 7)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n))
+    srv = cother_amd.Server(ctx, nslots=args.serve_slots, mem_budget=64 << 30, chunk_cols=4 << 20, eval=args.eval)
+    tickets = [None] * n
+    ready = [threading.Event() for _ in range(n)]
+
+    def submitter():
+        t0 = time.perf_counter()
+        for k, qi in enumerate(order):
+            dt = arrivals[k] - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            tickets[k] = srv.submit(qi)
+            ready[k].set()
+
+    log(f"serving: {n} requests, Poisson at {rate:.3f}/s ({args.serve_load}x the sequential rate), "
+        f"{args.serve_slots} slots")
+    t_start = time.perf_counter()
+    th = threading.Thread(target=submitter)
+    th.start()
+    lat, queued, cells = [], [], 0.0
+    for k in range(n):
+        ready[k].wait()
+        _, _, info = srv.wait(tickets[k])
+        lat.append(info["ms_latency"])
+        queued.append(info["ms_queued"])
+        cells += info["stats"]["cells"]
+        log(f"request {k}: query {order[k]} Lq={ql[order[k]]} latency {info['ms_latency']:.0f} ms "
+            f"(queued {info['ms_queued']:.0f} ms) slot {info['slot']}")
+    th.join()
+    makespan = time.perf_counter() - t_start
+    srv.close()
+    lat = np.array(lat)
+    return {"requests": n, "slots": args.serve_slots, "offered_qps": round(rate, 4),
+            "gcups": round(cells / makespan / 1e9, 4), "p50_ms": round(float(np.percentile(lat, 50)), 2),
+            "p99_ms": round(float(np.percentile(lat, 99)), 2), "mean_queued_ms": round(float(np.mean(queued)), 2),
+            "mem_budget_gib": 64, "chunk_cols": 4 << 20}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,6 +168,11 @@ def main():
     ap.add_argument("--eval", type=float, default=10.0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample size in seconds (0: skip)")
     ap.add_argument("--seed", type=int, default=20260)
+    ap.add_argument("--serve-requests", type=int, default=16,
+                    help="serving phase: requests (the timed queries again) arriving as a Poisson process (0: skip)")
+    ap.add_argument("--serve-load", type=float, default=1.0,
+                    help="serving phase: offered load as a multiple of the sequential query rate")
+    ap.add_argument("--serve-slots", type=int, default=2)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel over BYTES (traffic-counter calibration) and exit")
     args = ap.parse_args()
@@ -193,6 +247,9 @@ def main():
                 "traffic": (bpc * cells_per_launch) if bpc is not None else None,
                 "kernel": "k_score", "avg_launch_ms": round(ms_score / max(nlaunch, 1.0), 3),
                 "ops_per_launch": ops / max(nlaunch, 1.0)}
+    serving = None
+    if args.serve_requests > 0:
+        serving = serve_phase(ctx, qs, db, args, args.steps / wall)
     cpu = None
     if args.cpu_seconds > 0:
         log("cpu baseline ...")
@@ -212,7 +269,7 @@ def main():
                      "map": round(ms_map, 1)},
         "passed_per_query": round(npass / args.steps, 1), "hits_per_query": round(nhits / args.steps, 1),
         "map_cells": map_cells,
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "serving": serving,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

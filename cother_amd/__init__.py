@@ -44,7 +44,7 @@ class SearchOpts(ctypes.Structure):
     """Mirror of cth_search_opts (include/cother_hip.h)."""
     _fields_ = [("eval", ctypes.c_float), ("ssemodel", ctypes.c_int), ("do_map", ctypes.c_int),
                 ("timing", ctypes.c_int), ("mem_budget", ctypes.c_uint64), ("minpp", ctypes.c_float),
-                ("reserved", ctypes.c_int), ("alnbuf", ctypes.c_void_p), ("alncap", ctypes.c_uint64)]
+                ("contraction", ctypes.c_int), ("alnbuf", ctypes.c_void_p), ("alncap", ctypes.c_uint64)]
 
 
 class SearchStats(ctypes.Structure):
@@ -136,8 +136,27 @@ def lib():
     L.cth_write_results_plain.argtypes = [P, ctypes.c_int, P, ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p,
                                           ctypes.c_uint64, ctypes.POINTER(OutInfo), ctypes.c_char_p, ctypes.c_char_p,
                                           ctypes.c_int]
+    L.cth_score_block_ex.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_void_p, ctypes.c_void_p]
+    L.cth_selftest_mfma.argtypes = [ctypes.POINTER(ctypes.c_int)]
+    L.cth_format_hit_json.argtypes = L.cth_format_hit_plain.argtypes
+    L.cth_format_results_json.argtypes = L.cth_format_results_plain.argtypes
+    L.cth_write_results_json.argtypes = L.cth_write_results_plain.argtypes
     L.cth_free_text.argtypes = [ctypes.c_void_p]
     L.cth_free_text.restype = None
+    L.cth_ctx_create_shared.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]
+    L.cth_ctx_shape.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_void_p,
+                                ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int64)]
+    L.cth_server_opts_init.argtypes = [ctypes.POINTER(ServerOpts)]
+    L.cth_server_opts_init.restype = None
+    L.cth_server_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(ServerOpts), ctypes.POINTER(ctypes.c_void_p)]
+    L.cth_server_footprint.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]
+    L.cth_server_submit.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]
+    L.cth_server_wait.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_int), ctypes.c_char_p, ctypes.c_uint64,
+                                  ctypes.POINTER(RequestInfo)]
+    L.cth_server_destroy.argtypes = [ctypes.c_void_p]
+    L.cth_server_destroy.restype = None
     _lib = L
     return L
 
@@ -310,6 +329,13 @@ def pset_desc(ps, k):
     return None if v is None else v.decode()
 
 
+def selftest_mfma():
+    """Number of mismatching accumulators of the MFMA lane-layout check (0 expected)."""
+    bad = ctypes.c_int()
+    _check(lib().cth_selftest_mfma(ctypes.byref(bad)), "cth_selftest_mfma")
+    return bad.value
+
+
 def search_space(qlen, dbsize, ndb):
     """(effective search space, length adjustment) of a query of qlen positions."""
     ss, dl = ctypes.c_uint64(), ctypes.c_uint64()
@@ -324,39 +350,41 @@ def _take_text(p):
         lib().cth_free_text(p)
 
 
-def format_hit_plain(qs, qidx, db, hit, rows, **info):
-    """(annotation, alignment record) bytes of one hit; rows: the five alignment rows."""
+def format_hit_plain(qs, qidx, db, hit, rows, fmt="plain", **info):
+    """(annotation, alignment record) bytes of one hit; rows: the five alignment rows;
+    fmt "plain" or "json"."""
     o = outinfo(**info)
     h = np.array(hit, HIT).reshape(1)
     buf = b"".join(r if isinstance(r, bytes) else r.encode() for r in rows)
     a, r = ctypes.c_void_p(), ctypes.c_void_p()
-    _check(lib().cth_format_hit_plain(qs.ptr, qidx, db.ptr, h.ctypes.data, buf, ctypes.byref(o), ctypes.byref(a),
-                                      ctypes.byref(r)), "cth_format_hit_plain")
+    fn = lib().cth_format_hit_json if fmt == "json" else lib().cth_format_hit_plain
+    _check(fn(qs.ptr, qidx, db.ptr, h.ctypes.data, buf, ctypes.byref(o), ctypes.byref(a), ctypes.byref(r)),
+           "cth_format_hit_" + fmt)
     return _take_text(a), _take_text(r)
 
 
-def format_results_plain(qs, qidx, db, hits, alnbuf, **info):
-    """The reference's plain-text results file for query qidx, as bytes."""
+def format_results_plain(qs, qidx, db, hits, alnbuf, fmt="plain", **info):
+    """The reference's results file for query qidx (fmt "plain" or "json"), as bytes."""
     o = outinfo(**info)
     h = np.ascontiguousarray(hits, HIT)
     t, n = ctypes.c_void_p(), ctypes.c_uint64()
-    _check(lib().cth_format_results_plain(qs.ptr, qidx, db.ptr, h.ctypes.data, len(h), alnbuf, len(alnbuf),
-                                          ctypes.byref(o), ctypes.byref(t), ctypes.byref(n)),
-           "cth_format_results_plain")
+    fn = lib().cth_format_results_json if fmt == "json" else lib().cth_format_results_plain
+    _check(fn(qs.ptr, qidx, db.ptr, h.ctypes.data, len(h), alnbuf, len(alnbuf), ctypes.byref(o), ctypes.byref(t),
+              ctypes.byref(n)), "cth_format_results_" + fmt)
     try:
         return ctypes.string_at(t, n.value)
     finally:
         lib().cth_free_text(t)
 
 
-def write_results_plain(qs, qidx, db, hits, alnbuf, outdir, **info):
-    """Write the results file of query qidx into outdir; returns its path."""
+def write_results_plain(qs, qidx, db, hits, alnbuf, outdir, fmt="plain", **info):
+    """Write the results file of query qidx into outdir (fmt "plain" or "json"); returns its path."""
     o = outinfo(**info)
     h = np.ascontiguousarray(hits, HIT)
     path = ctypes.create_string_buffer(4096)
-    _check(lib().cth_write_results_plain(qs.ptr, qidx, db.ptr, h.ctypes.data, len(h), alnbuf, len(alnbuf),
-                                         ctypes.byref(o), outdir.encode(), path, len(path)),
-           "cth_write_results_plain")
+    fn = lib().cth_write_results_json if fmt == "json" else lib().cth_write_results_plain
+    _check(fn(qs.ptr, qidx, db.ptr, h.ctypes.data, len(h), alnbuf, len(alnbuf), ctypes.byref(o), outdir.encode(),
+              path, len(path)), "cth_write_results_" + fmt)
     return path.value.decode()
 
 class Context:
@@ -384,13 +412,16 @@ class Context:
         _check(lib().cth_upload_db(self.h, db.ptr), "cth_upload_db")
         self.db = db
 
-    def score_block(self, qidx, p0, p1):
+    def score_block(self, qidx, p0, p1, contraction=0):
+        """Score and modular-score matrices [Lq][ncols]; contraction 1/2: log-odds term from the
+        BF16/FP16 matrix-core product (0: FP32, the reference's)."""
         lq = int(self.qs.lens()[qidx])
         d, ln = self.db.dists(), self.db.lens()
         ncols = int(d[p1 - 1] + ln[p1 - 1] - d[p0])
         sc = np.empty((lq, ncols), np.float32)
         md = np.empty((lq, ncols), np.float32)
-        _check(lib().cth_score_block(self.h, qidx, p0, p1, sc.ctypes.data, md.ctypes.data), "cth_score_block")
+        _check(lib().cth_score_block_ex(self.h, qidx, p0, p1, contraction, sc.ctypes.data, md.ctypes.data),
+               "cth_score_block_ex")
         return sc, md
 
     def phase1_block(self, qidx, p0, p1, scorethld, want_btck=False):
@@ -429,7 +460,7 @@ class Context:
         return res
 
     def search(self, q0=0, q1=None, eval=10.0, ssemodel=2, do_map=True, timing=False, mem_budget=0,
-               maxhits=None, minpp=MINPP, alignments=False):
+               maxhits=None, minpp=MINPP, alignments=False, contraction=0):
         """Search queries [q0,q1) against the resident database.
 
         Returns (hits HIT array grouped by query, per-query hit counts, SearchStats), and with
@@ -446,7 +477,7 @@ class Context:
         if alignments:
             cap = 5 * min(maxhits, 4096) * int(self.qs.lens()[q0:q1].max() + self.db.lens().max())
             abuf = ctypes.create_string_buffer(cap)
-        opts = SearchOpts(eval, ssemodel, int(do_map), int(timing), mem_budget, minpp, 0,
+        opts = SearchOpts(eval, ssemodel, int(do_map), int(timing), mem_budget, minpp, contraction,
                           ctypes.cast(abuf, ctypes.c_void_p) if abuf is not None else None,
                           len(abuf) if abuf is not None else 0)
         _check(lib().cth_search(self.h, q0, q1, ctypes.byref(opts), hits.ctypes.data, maxhits, nh.ctypes.data,
@@ -460,3 +491,68 @@ class Context:
             n, o = int(h["alnlen"]), int(h["alnoff"])
             alns.append(tuple(raw[o + r * n:o + (r + 1) * n].decode() for r in range(5)))
         return hits, nh, st, alns
+
+class ServerOpts(ctypes.Structure):
+    """Mirror of cth_server_opts (include/cother_hip.h)."""
+    _fields_ = [("nslots", ctypes.c_int), ("mem_budget", ctypes.c_uint64), ("chunk_cols", ctypes.c_uint64),
+                ("search", SearchOpts), ("alncap", ctypes.c_uint64)]
+
+
+class RequestInfo(ctypes.Structure):
+    """Mirror of cth_request_info (include/cother_hip.h)."""
+    _fields_ = [("qidx", ctypes.c_int32), ("slot", ctypes.c_int32), ("footprint", ctypes.c_uint64),
+                ("ms_queued", ctypes.c_double), ("ms_latency", ctypes.c_double), ("ms_service", ctypes.c_double),
+                ("stats", SearchStats)]
+
+
+class Server:
+    """The serving loop (cth_server_*) over a context's uploaded queries and resident database."""
+
+    def __init__(self, ctx, nslots=2, mem_budget=64 << 30, chunk_cols=4 << 20, eval=10.0, ssemodel=2, do_map=True,
+                 minpp=MINPP, alncap=0):
+        o = ServerOpts()
+        lib().cth_server_opts_init(ctypes.byref(o))
+        o.nslots, o.mem_budget, o.chunk_cols, o.alncap = nslots, mem_budget, chunk_cols, alncap
+        o.search.eval, o.search.ssemodel, o.search.do_map, o.search.minpp = eval, ssemodel, int(do_map), minpp
+        self.ctx = ctx
+        self.ndb = ctx.db.nprof
+        self.alncap = alncap
+        h = ctypes.c_void_p()
+        _check(lib().cth_server_create(ctx.h, ctypes.byref(o), ctypes.byref(h)), "cth_server_create")
+        self.h = h
+
+    def footprint(self, qidx):
+        b = ctypes.c_uint64()
+        _check(lib().cth_server_footprint(self.h, qidx, ctypes.byref(b)), "cth_server_footprint")
+        return b.value
+
+    def submit(self, qidx):
+        t = ctypes.c_uint64()
+        _check(lib().cth_server_submit(self.h, qidx, ctypes.byref(t)), "cth_server_submit")
+        return t.value
+
+    def wait(self, ticket):
+        """(hits, alignments or None, info dict) of a submitted request."""
+        hits = np.zeros(max(1, self.ndb), HIT)
+        n = ctypes.c_int()
+        buf = ctypes.create_string_buffer(self.alncap) if self.alncap else None
+        info = RequestInfo()
+        _check(lib().cth_server_wait(self.h, ticket, hits.ctypes.data, len(hits), ctypes.byref(n), buf,
+                                     self.alncap, ctypes.byref(info)), "cth_server_wait")
+        hits = hits[:n.value]
+        alns = None
+        if buf is not None:
+            raw = buf.raw
+            alns = [tuple(raw[int(h["alnoff"]) + r * int(h["alnlen"]):int(h["alnoff"]) + (r + 1) * int(h["alnlen"])]
+                          .decode() for r in range(5)) for h in hits]
+        d = {k: getattr(info, k) for k, _ in RequestInfo._fields_ if k != "stats"}
+        d["stats"] = info.stats.asdict()
+        return hits, alns, d
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cth_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

@@ -72,13 +72,54 @@ extern "C" int cth_ctx_create(int device, const char* envdir, cth_ctx** out) {
   return 0;
 }
 
+// A context on the parent's device with its own stream and scratch that
+// reads the parent's tables, query set and resident database (as the
+// reference's device-memory areas of one CuDeviceMemory share its cached
+// data, CuDeviceMemory.cuh:148-154).  The parent must outlive it, and
+// uploads to the parent after this call are not seen by it.
+extern "C" int cth_ctx_create_shared(cth_ctx* parent, cth_ctx** out) {
+  *out = nullptr;
+  if (!parent || parent->borrowed) { cth_set_error("shared context: invalid parent"); return -1; }
+  CTH_HIP_CHECK(hipSetDevice(parent->device));
+  cth_ctx* c = (cth_ctx*)calloc(1, sizeof(cth_ctx));
+  if (!c) { cth_set_error("out of host memory"); return -1; }
+  c->device = parent->device;
+  c->borrowed = 1;
+  c->htab = parent->htab;
+  c->dtab = parent->dtab;
+  c->q = parent->q;
+  c->db = parent->db;
+  c->has_q = parent->has_q;
+  c->has_db = parent->has_db;
+  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+    cth_set_error("hipStreamCreate failed"); free(c); return -1;
+  }
+  *out = c;
+  return 0;
+}
+
+extern "C" void cth_ctx_trim(cth_ctx* c, size_t keep) {
+  struct { void** p; size_t* cap; } bufs[] = {
+      {(void**)&c->scores, &c->cap_scores}, {(void**)&c->mods, &c->cap_mods}, {(void**)&c->btck, &c->cap_btck},
+      {(void**)&c->mapbuf, &c->cap_mapbuf}, {&c->work, &c->cap_work}};
+  for (auto& b : bufs)
+    if (*b.p && *b.cap > keep) {
+      (void)hipFree(*b.p);
+      *b.p = nullptr;
+      *b.cap = 0;
+    }
+}
+
 extern "C" void cth_ctx_destroy(cth_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->st) (void)hipStreamSynchronize(c->st);
-  cth_devset_free(&c->q);
-  cth_devset_free(&c->db);
-  void* bufs[] = {c->dtab, c->scores, c->mods, c->btck, c->dpout, c->work, c->mapbuf};
+  if (!c->borrowed) {
+    cth_devset_free(&c->q);
+    cth_devset_free(&c->db);
+    if (c->dtab) (void)hipFree(c->dtab);
+  }
+  void* bufs[] = {c->scores, c->mods, c->btck, c->dpout, c->work, c->mapbuf};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (c->h_pin) (void)hipHostFree(c->h_pin);
@@ -88,16 +129,30 @@ extern "C" void cth_ctx_destroy(cth_ctx* c) {
   free(c);
 }
 
+// the uploaded sets' sizes: number of queries and their lengths (host copy
+// owned by the context), database profiles and positions
+extern "C" int cth_ctx_shape(const cth_ctx* c, int* nqueries, const int32_t** qlens, int* ndb, int64_t* dbpos) {
+  if (!c->has_q || !c->has_db) { cth_set_error("queries or database not uploaded"); return -1; }
+  if (nqueries) *nqueries = c->q.d.nprof;
+  if (qlens) *qlens = c->q.h_len;
+  if (ndb) *ndb = c->db.d.nprof;
+  if (dbpos) *dbpos = c->db.d.npos;
+  return 0;
+}
+
 extern "C" int cth_upload_queries(cth_ctx* c, const cth_pset* qs) {
+  if (c->borrowed) { cth_set_error("upload on a shared context"); return -1; }
   CTH_HIP_CHECK(hipSetDevice(c->device));
   cth_devset_free(&c->q);
   c->has_q = 0;
   if (cth_devset_upload(qs, &c->q)) return -1;
+  if (cth_devset_make_ddoff(qs, &c->q)) { cth_devset_free(&c->q); return -1; }
   c->has_q = 1;
   return 0;
 }
 
 extern "C" int cth_upload_db(cth_ctx* c, const cth_pset* db) {
+  if (c->borrowed) { cth_set_error("upload on a shared context"); return -1; }
   CTH_HIP_CHECK(hipSetDevice(c->device));
   cth_devset_free(&c->db);
   c->has_db = 0;
@@ -106,13 +161,15 @@ extern "C" int cth_upload_db(cth_ctx* c, const cth_pset* db) {
   return 0;
 }
 
-extern "C" int cth_launch_score(const cth_dpset* q, int qp, const cth_dpset* db, int64_t col0, int ncols,
-                                int64_t ldsc, const cth_tables* dtabs, int lq, float* scores, float* mods,
+extern "C" int cth_launch_score(const cth_dpset* q, const int32_t* qoff, int qp, const cth_dpset* db, int64_t col0, int ncols,
+                                int64_t ldsc, const cth_tables* dtabs, int lq, float* scores, float* mods, int contraction,
                                 hipStream_t st);
 
 // Score matrix of query qidx against template profiles [p0,p1), copied to
 // host arrays [Lq][ncols] (ncols = positions of those profiles).
-extern "C" int cth_score_block(cth_ctx* c, int qidx, int p0, int p1, float* h_scores, float* h_mods) {
+extern "C" int cth_score_block_ex(cth_ctx* c, int qidx, int p0, int p1, int contraction, float* h_scores,
+                                  float* h_mods) {
+  if (contraction < 0 || contraction > 2) { cth_set_error("score_block: contraction %d", contraction); return -1; }
   if (!c->has_q || !c->has_db) { cth_set_error("score_block: queries or database not uploaded"); return -1; }
   if (qidx < 0 || qidx >= c->q.d.nprof) { cth_set_error("score_block: query %d out of range", qidx); return -1; }
   if (p0 < 0 || p1 > c->db.d.nprof || p0 >= p1) { cth_set_error("score_block: bad profile range [%d,%d)", p0, p1); return -1; }
@@ -125,7 +182,8 @@ extern "C" int cth_score_block(cth_ctx* c, int qidx, int p0, int p1, float* h_sc
   const size_t need = sizeof(float) * (size_t)lq * ldsc;
   if (cth_ctx_reserve(c, (void**)&c->scores, &c->cap_scores, need)) return -1;
   if (cth_ctx_reserve(c, (void**)&c->mods, &c->cap_mods, need)) return -1;
-  if (cth_launch_score(&c->q.d, qidx, &c->db.d, col0, (int)ncols, ldsc, c->dtab, lq, c->scores, c->mods, c->st))
+  if (cth_launch_score(&c->q.d, c->q.ddoff, qidx, &c->db.d, col0, (int)ncols, ldsc, c->dtab, lq, c->scores, c->mods,
+                       contraction, c->st))
     return -1;
   CTH_HIP_CHECK(hipStreamSynchronize(c->st));
   CTH_HIP_CHECK(hipMemcpy2D(h_scores, sizeof(float) * ncols, c->scores, sizeof(float) * ldsc, sizeof(float) * ncols,
@@ -133,6 +191,10 @@ extern "C" int cth_score_block(cth_ctx* c, int qidx, int p0, int p1, float* h_sc
   CTH_HIP_CHECK(hipMemcpy2D(h_mods, sizeof(float) * ncols, c->mods, sizeof(float) * ldsc, sizeof(float) * ncols,
                             lq, hipMemcpyDeviceToHost));
   return 0;
+}
+
+extern "C" int cth_score_block(cth_ctx* c, int qidx, int p0, int p1, float* h_scores, float* h_mods) {
+  return cth_score_block_ex(c, qidx, p0, p1, 0, h_scores, h_mods);
 }
 
 extern "C" int cth_launch_dp(const cth_dpset* q, int qp, const cth_dpset* db, int p0, int np, int ltmax,
@@ -160,7 +222,7 @@ extern "C" int cth_phase1_block(cth_ctx* c, int qidx, int p0, int p1, float scor
   if (cth_ctx_reserve(c, (void**)&c->mods, &c->cap_mods, need)) return -1;
   if (cth_ctx_reserve(c, (void**)&c->btck, &c->cap_btck, (size_t)lq * ldsc)) return -1;
   if (cth_ctx_reserve(c, (void**)&c->dpout, &c->cap_dpout, sizeof(cth_dpres) * (size_t)(p1 - p0))) return -1;
-  if (cth_launch_score(&c->q.d, qidx, &c->db.d, col0, (int)ncols, ldsc, c->dtab, lq, c->scores, c->mods, c->st))
+  if (cth_launch_score(&c->q.d, c->q.ddoff, qidx, &c->db.d, col0, (int)ncols, ldsc, c->dtab, lq, c->scores, c->mods, 0, c->st))
     return -1;
   if (cth_launch_dp(&c->q.d, qidx, &c->db.d, p0, p1 - p0, ltmax, col0, ldsc, c->scores, c->btck, scorethld,
                     (cth_dpres*)c->dpout, c->st))
@@ -258,7 +320,7 @@ extern "C" int cth_map_block(cth_ctx* c, int qidx, int p0, int p1, const int32_t
   cth_mapjob* d_jobs = (cth_mapjob*)(d_zrow + (size_t)njobs * lq);
   cth_mapres* d_res = (cth_mapres*)(((uintptr_t)(d_jobs + njobs) + 15) & ~(uintptr_t)15);
   CTH_HIP_CHECK(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(cth_mapjob) * njobs, hipMemcpyHostToDevice, c->st));
-  if (cth_launch_score(&c->q.d, qidx, &c->db.d, col0, (int)ncols, ldsc, c->dtab, lq, c->scores, c->mods, c->st))
+  if (cth_launch_score(&c->q.d, c->q.ddoff, qidx, &c->db.d, col0, (int)ncols, ldsc, c->dtab, lq, c->scores, c->mods, 0, c->st))
     return -1;
   if (cth_launch_map(&c->q.d, qidx, lq, &c->db.d, d_jobs, njobs, ltmax, col0, ldsc, c->scores, c->mods,
                      (float*)c->mapbuf, c->btck, d_zrow, minpp, nullptr, d_res, c->st))

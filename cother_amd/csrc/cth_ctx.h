@@ -24,7 +24,13 @@ struct cth_ctx {
   void* h_pin; size_t cap_pin;
   // stage events of cth_search's timing mode (created on first use)
   hipEvent_t ev[4]; int has_ev;
+  // a context made by cth_ctx_create_shared reads its parent's tables,
+  // queries and database and frees only its own stream and scratch
+  int borrowed;
 };
+
+// release scratch buffers larger than keep bytes (the next allocation sizes them anew)
+extern "C" void cth_ctx_trim(cth_ctx* c, size_t keep);
 
 int cth_ctx_reserve(cth_ctx* c, void** p, size_t* cap, size_t need);
 extern "C" void cth_ctx_destroy(cth_ctx* c);

@@ -8,7 +8,12 @@ struct cth_devset {
   cth_dpset d;        // device pointers (kernel argument)
   int32_t* h_len;     // host copy of profile lengths
   int64_t* h_dist;    // host copy of profile offsets
+  int32_t* ddoff;     // query sets: [npos][128] distance lists as score-table byte offsets
+                      // (entry 0 = count, entry k = 22 * 4 * value k), read by the score kernel
+                      // with scalar loads
 };
+
+extern "C" int cth_devset_make_ddoff(const cth_pset* h, cth_devset* s);
 
 extern "C" int cth_devset_upload(const cth_pset* h, cth_devset* s);
 extern "C" void cth_devset_free(cth_devset* s);

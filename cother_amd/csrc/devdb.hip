@@ -46,6 +46,7 @@ extern "C" void cth_devset_free(cth_devset* s) {
                   (void*)s->d.pro, (void*)s->d.trg, (void*)s->d.trn, (void*)s->d.ss, (void*)s->d.cv,
                   (void*)s->d.cvprior, (void*)s->d.cvnorm2, (void*)s->d.dd})
     if (p) (void)hipFree(p);
+  if (s->ddoff) (void)hipFree(s->ddoff);
   free(s->h_len);
   free(s->h_dist);
   memset(s, 0, sizeof *s);
@@ -112,5 +113,28 @@ extern "C" int cth_devset_upload(const cth_pset* h, cth_devset* s) {
     memcpy(s->h_dist, h->dist, sizeof(int64_t) * h->nprof);
   }
   if (rc) { cth_devset_free(s); return -1; }
+  return 0;
+}
+
+// The query set's distance lists as byte offsets into the 22 x 22 pair-score
+// table (value v -> row v, 22 floats per row), one 128-entry record per
+// position, so the score kernel adds the template value's column directly.
+extern "C" int cth_devset_make_ddoff(const cth_pset* h, cth_devset* s) {
+  const int64_t n = h->npos;
+  std::vector<int32_t> buf((size_t)std::max<int64_t>(n, 1) * CTH_DDSTRIDE);
+  for (int64_t i = 0; i < n; i++) {
+    const uint8_t* r = h->dd + (size_t)i * CTH_DDSTRIDE;
+    int32_t* o = &buf[(size_t)i * CTH_DDSTRIDE];
+    o[0] = r[0];
+    for (int k = 1; k < CTH_DDSTRIDE; k++) {
+      const uint8_t v = r[k];
+      const int c = v == 255 ? CTH_DD_MISSING : (v < CTH_DMS_MAXDST ? v : CTH_DMS_MAXDST);
+      o[k] = c * 22 * (int)sizeof(float);
+    }
+  }
+  if (s->ddoff) (void)hipFree(s->ddoff);
+  s->ddoff = nullptr;
+  CTH_HIP_CHECK(hipMalloc((void**)&s->ddoff, buf.size() * sizeof(int32_t)));
+  CTH_HIP_CHECK(hipMemcpy(s->ddoff, buf.data(), buf.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   return 0;
 }

@@ -12,6 +12,14 @@
 // records, reference statistics.  Descriptions are wrapped by the reference's
 // word-wrapping rule (liblib/fmtdescription.h:35-83), restated here as
 // wrap_text() because its line breaks and truncation are part of the format.
+//
+// The JSON format follows the same two stages (CompressResultsJSON,
+// cupro/CuBatchProcessingFinalizerJSON.cpp:35-383; AlnWriter::
+// WriteResultsJSON, cualn/AlnWriterJSON.cpp:19-429), with the reference's
+// escaping rule for strings (FormatDescriptionJSON, liblib/fmtdescription.h:
+// 94-128) restated as json_text().  Kept as the reference writes it: the
+// separator after the last alignment record is chosen by NOHITS, not NOALNS
+// (AlnWriterJSON.cpp:112-118), so NOALNS < NOHITS leaves a trailing comma.
 #include <stdarg.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -99,6 +107,33 @@ void wrap_text(std::string& out, const char* text, int maxlen, int indent, int w
   }
   if (maxlen <= outpos && 3 < maxlen)
     for (size_t k = 1; k <= 3 && k <= out.size(); k++) out[out.size() - k] = '.';
+}
+
+// The JSON string rule: leading white space dropped, each run of blanks one
+// space, '"', '\\' and '/' escaped with a backslash; output stops once `outpos`
+// reaches `maxlen`, and then the last three characters, and any backslashes
+// before them, become '.'.
+void json_text(std::string& out, const char* text, int maxlen, int& outpos) {
+  while (*text == ' ' || *text == '\t' || *text == '\n' || *text == '\r') text++;
+  while (*text && outpos < maxlen) {
+    if (blank(*text)) {
+      while (blank(*text)) text++;
+      out += ' ';
+      outpos++;
+      continue;
+    } else if (*text == '"' || *text == '\\' || *text == '/') {
+      out += '\\';
+      outpos++;
+      if (maxlen <= outpos) break;
+    }
+    out += *text++;
+    outpos++;
+  }
+  if (maxlen <= outpos && 3 < maxlen) {
+    int i = 1;
+    for (; i <= 3 && i <= (int)out.size(); i++) out[out.size() - i] = '.';
+    for (; i <= outpos && i <= (int)out.size() && out[out.size() - i] == '\\'; i++) out[out.size() - i] = '.';
+  }
 }
 
 inline unsigned coord_x(uint32_t xy) { return (xy >> 16) & 0xffff; }
@@ -354,8 +389,159 @@ char* to_c(const std::string& s, uint64_t* len) {
   return p;
 }
 
+// MakeAnnotationJSON (CuBatchProcessingFinalizerJSON.cpp:161-187)
+void annotation_json(std::string& out, const char* desc, float score, double evalue) {
+  out += "    {\"summary_entry\": {\n      \"description\": \"";
+  int outpos = 0;
+  if (desc) json_text(out, desc, kAnnotLen, outpos);
+  out += "\",\n";
+  appendf(out, "      \"score\": %.0f,\n      \"evalue\": %.2g\n    }}", score, evalue);
+}
+
+// The JSON alignment record of one hit (CompressResultsJSON with
+// FormatScoresJSON, FormatAlignmentJSON and FormatFooterJSON,
+// CuBatchProcessingFinalizerJSON.cpp:116-145, 197-383)
+void record_json(std::string& out, const Meta& m, const cth_pset* db, const cth_hit& h, const char* rows,
+                 const cth_outinfo& o) {
+  const int p = h.profile;
+  const char* desc = cth_pset_desc(db, p);
+  out += "    {\"hit_record\": {\n      \"target_description\": \"";
+  int outpos = 0;
+  if (desc) json_text(out, desc, o.dsclen, outpos);
+  out += "\",\n";
+  appendf(out,
+          "      \"query_length\": %d,\n      \"query_eno\": %.1f,\n      \"target_length\": %d,\n"
+          "      \"target_eno\": %.1f,\n      \"alignment\": {\n",
+          m.qlen, m.qeno, db->len[p], db->eno[p]);
+  const double evalue = evalue_of(h.logE);
+  appendf(out, "        \"score\": %.2f,\n        \"bit_score\": %.1f,\n        \"evalue\": %.2g,\n"
+               "        \"pvalue\": %.2g,\n",
+          h.score, bits_of(m.sspace, h.logE), evalue, pvalue_of(evalue));
+  const unsigned alnlen = (unsigned)h.alnlen;
+  appendf(out, "        \"n_identities\": %d,\n        \"n_positives\": %d,\n        \"n_gaps\": %d,\n"
+               "        \"aln_length\": %d,\n",
+          h.idnts, h.pstvs_map, h.ngaps, (int)alnlen);
+  const char* rq = rows;
+  const char* rm = rows + alnlen;
+  const char* rt = rows + 2 * (size_t)alnlen;
+  const unsigned qbeg = coord_y(h.mapbeg) + 1, tbeg = coord_x(h.mapbeg) + 1;
+  const int n = (int)alnlen;
+  const int gq = (int)std::count(rq, rq + n, '-'), gt = (int)std::count(rt, rt + n, '-');
+  const unsigned qend = qbeg + n - gq, tend = tbeg + n - gt;
+  // SS strings whenever both profiles have SS predictions (the JSON writer does not consult SSSWGT)
+  const int ss = m.qss && has_ss(db, p);
+  appendf(out, "        \"query_from\": %u,\n        \"query_to\": %u,\n        \"target_from\": %u,\n"
+               "        \"target_to\": %u,\n",
+          qbeg, n <= gq ? qend : qend - 1, tbeg, n <= gt ? tend : tend - 1);
+  appendf(out, "        \"query_secstr_available\": %d,\n        \"target_secstr_available\": %d,\n"
+               "        \"query_secstr\": \"",
+          ss, ss);
+  if (ss) out.append(rows + 3 * (size_t)alnlen, n);
+  out += "\",\n        \"target_secstr\": \"";
+  if (ss) out.append(rows + 4 * (size_t)alnlen, n);
+  out += "\",\n        \"query_aln\": \"";
+  out.append(rq, n);
+  out += "\",\n        \"target_aln\": \"";
+  out.append(rt, n);
+  out += "\",\n        \"middle\": \"";
+  out.append(rm, n);
+  out += "\",\n";
+  auto val = [](float v, bool ok) {
+    char b[32];
+    if (ok && v > 0.0f) snprintf(b, sizeof b, "%6.4f", v);
+    else snprintf(b, sizeof b, "NA");
+    return std::string(b);
+  };
+  const bool ok = h.expected < 0.0f;
+  appendf(out, "        \"computed_K_ungapped\": \"%s\",\n        \"computed_lambda_ungapped\": \"%s\",\n",
+          val(h.K, ok).c_str(), val(h.lambda, ok).c_str());
+  appendf(out,
+          "        \"estimated_K_gapped\": \"%s\",\n        \"estimated_lambda_gapped\": \"%s\",\n"
+          "        \"entropy\": %.4f,\n        \"expected_score\": %.4f,\n        \"min_score\": %.0f,\n"
+          "        \"max_score\": %.0f\n",
+          val(h.K_est, ok).c_str(), val(h.lambda_est, ok).c_str(), h.H, h.expected, h.minsc, h.maxsc);
+  out += "      }\n    }}";
+}
+
+// WritePrognameJSON (AlnWriterJSON.cpp:154-201); an empty reference list
+// gives one empty string, as the reference's does
+void progname_json(std::string& out, const cth_outinfo& o) {
+  appendf(out, "{\"cother_search\": {\n  \"program\": \"%s\",\n  \"version\": \"%s\",\n",
+          o.progname ? o.progname : "", o.progversion ? o.progversion : "");
+  if (kWriterBuffer - (int)out.size() - 1024 < 1) return;
+  out += "  \"references\": [\n    \"\"\n  ],\n";
+}
+
+// WriteQueryDescriptionJSON (AlnWriterJSON.cpp:215-266)
+void query_description_json(std::string& out, int maxsize, const Meta& m) {
+  if (maxsize < 60) return;
+  const size_t start = out.size();
+  appendf(out, "  \"query\": {\n    \"length\": %d,\n", m.qlen);
+  maxsize -= (int)(out.size() - start) + 256;  // BUF_MAX (liblib/mylimits.h:41)
+  if (maxsize < 1) return;
+  int outpos = 0;
+  const char* name_open = "    \"name\": \"";
+  out += name_open;
+  outpos += (int)strlen(name_open);
+  const char* close = "\",\n";
+  out += close;
+  outpos += (int)strlen(close);
+  const char* desc_open = "    \"description\": \"";
+  out += desc_open;
+  outpos += (int)strlen(desc_open);
+  if (m.qdesc) json_text(out, m.qdesc, maxsize, outpos);
+  out += "\"\n  },\n";
+}
+
+// WriteSearchInformationJSON (AlnWriterJSON.cpp:281-360) in a 1 KiB budget
+void search_information_json(std::string& out, const cth_outinfo& o, bool found) {
+  int maxsize = kTmpBuffer;
+  auto full = [&](int delta) {
+    maxsize -= delta;
+    return maxsize < 1;
+  };
+  if (full(40 * 7)) return;
+  out += "  \"database\": {\n    \"names\": [\n";
+  for (int k = 0; k < o.ndbnames; k++) {
+    const std::string name = base_name(o.dbnames[k]);
+    const size_t n = std::min<size_t>(name.size(), 99);
+    if (full((int)n + 15)) {
+      out += "      \"...\"\n";
+      break;
+    }
+    out += "      \"";
+    int outpos = 0;
+    json_text(out, name.c_str(), (int)n + 1, outpos);
+    out += (k + 1 < o.ndbnames) ? "\",\n" : "\"\n";
+  }
+  appendf(out, "    ],\n    \"number_of_profiles\": %llu,\n    \"number_of_positions\": %llu\n  },\n",
+          (unsigned long long)o.ndbentries, (unsigned long long)o.dbsize);
+  if (full(90 * 4)) return;
+  if (found) out += "  \"message\": \"Profiles found below the e-value threshold:\",\n";
+  else
+    appendf(out, "  \"message\": \"No profiles found below an e-value threshold of %g\",\n", exp(o.logevthld));
+}
+
+// WriteSummaryJSON (AlnWriterJSON.cpp:375-429)
+void summary_json(std::string& out, const Meta& m, const cth_outinfo& o) {
+  int effq = m.qlen;
+  uint64_t effdb = o.dbsize;
+  const int dl = (int)m.deltalen;
+  if (dl < m.qlen) effq -= dl;
+  if (o.ndbentries * (uint64_t)dl < o.dbsize) effdb -= o.ndbentries * (uint64_t)dl;
+  appendf(out,
+          "  \"search_statistics\": {\n    \"reference_K_ungapped\": %.4f,\n"
+          "    \"reference_lambda_ungapped\": %.4f,\n    \"reference_K_gapped\": %.4f,\n"
+          "    \"reference_lambda_gapped\": %.4f,\n    \"query_length\": %d,\n    \"database_size\": %llu,\n"
+          "    \"effective_query_length\": %d,\n    \"effective_database_size\": %llu,\n"
+          "    \"effective_search_space\": %llu\n  }\n}}\n",
+          o.ref_k, o.ref_lambda, o.gap_k, o.gap_lambda, m.qlen, (unsigned long long)o.dbsize, effq,
+          (unsigned long long)effdb, (unsigned long long)(size_t)m.sspace);
+}
+
 bool format_file(const cth_pset* qs, int qidx, const cth_pset* db, const cth_hit* hits, int nhits,
-                 const char* alnbuf, uint64_t alncap, const cth_outinfo* o, std::string& out, Meta& m) {
+                 const char* alnbuf, uint64_t alncap, const cth_outinfo* o, std::string& out, Meta& m,
+                 bool json = false) {
   if (!db || !o) {
     cth_set_error("results: null database or options");
     return false;
@@ -367,6 +553,30 @@ bool format_file(const cth_pset* qs, int qidx, const cth_pset* db, const cth_hit
   if (!make_meta(qs, qidx, o, m)) return false;
   const std::vector<int> idx = reported(hits, nhits, qidx, *o);
   if (!check_hits(db, hits, idx, alnbuf, alncap)) return false;
+  if (json) {
+    // AlnWriter::WriteResultsJSON (AlnWriterJSON.cpp:19-146)
+    progname_json(out, *o);
+    const int left = std::min<int>(kWriterBuffer - (int)out.size(), o->dsclen);
+    query_description_json(out, left, m);
+    const bool found = !idx.empty();
+    search_information_json(out, *o, found);
+    out += "  \"search_summary\": [\n";
+    const size_t n = idx.size(), nh = (size_t)std::max(0, o->nohits), na = (size_t)std::max(0, o->noalns);
+    for (size_t i = 0; i < n && i < nh; i++) {
+      const cth_hit& h = hits[idx[i]];
+      annotation_json(out, cth_pset_desc(db, h.profile), h.score, evalue_of(h.logE));
+      out += (i + 1 < n && i + 1 < nh) ? ",\n" : "\n";
+    }
+    out += "  ],\n  \"search_hits\": [\n";
+    for (size_t i = 0; i < n && i < na; i++) {
+      const cth_hit& h = hits[idx[i]];
+      record_json(out, m, db, h, alnbuf + h.alnoff, *o);
+      out += (i + 1 < n && i + 1 < nh) ? ",\n" : "\n";
+    }
+    out += "  ],\n";
+    summary_json(out, m, *o);
+    return true;
+  }
   // AlnWriter::WriteResultsPlain (AlnWriterPlain.cpp:19-121)
   progname(out, *o);
   const int left = std::min<int>(kWriterBuffer - (int)out.size(), o->dsclen);
@@ -408,8 +618,10 @@ extern "C" void cth_outinfo_init(cth_outinfo* o) {
   o->progversion = "2.03.03";  // cother/cother.h:23
 }
 
-extern "C" int cth_format_hit_plain(const cth_pset* qs, int qidx, const cth_pset* db, const cth_hit* hit,
-                                    const char* rows, const cth_outinfo* o, char** annot, char** rec) {
+namespace {
+
+int format_hit(const cth_pset* qs, int qidx, const cth_pset* db, const cth_hit* hit, const char* rows,
+               const cth_outinfo* o, char** annot, char** rec, bool json) {
   Meta m;
   if (!db || !o || !hit || !rows) {
     cth_set_error("results: null argument");
@@ -421,8 +633,13 @@ extern "C" int cth_format_hit_plain(const cth_pset* qs, int qidx, const cth_pset
     return -1;
   }
   std::string a, r;
-  annotation(a, cth_pset_desc(db, hit->profile), hit->score, evalue_of(hit->logE));
-  record(r, m, db, *hit, rows, *o);
+  if (json) {
+    annotation_json(a, cth_pset_desc(db, hit->profile), hit->score, evalue_of(hit->logE));
+    record_json(r, m, db, *hit, rows, *o);
+  } else {
+    annotation(a, cth_pset_desc(db, hit->profile), hit->score, evalue_of(hit->logE));
+    record(r, m, db, *hit, rows, *o);
+  }
   *annot = to_c(a, nullptr);
   *rec = to_c(r, nullptr);
   if (!*annot || !*rec) {
@@ -433,31 +650,30 @@ extern "C" int cth_format_hit_plain(const cth_pset* qs, int qidx, const cth_pset
   return 0;
 }
 
-extern "C" int cth_format_results_plain(const cth_pset* qs, int qidx, const cth_pset* db, const cth_hit* hits,
-                                        int nhits, const char* alnbuf, uint64_t alncap, const cth_outinfo* o,
-                                        char** text, uint64_t* len) {
+int format_results(const cth_pset* qs, int qidx, const cth_pset* db, const cth_hit* hits, int nhits,
+                   const char* alnbuf, uint64_t alncap, const cth_outinfo* o, char** text, uint64_t* len, bool json) {
   std::string out;
   Meta m;
-  if (!format_file(qs, qidx, db, hits, nhits, alnbuf, alncap, o, out, m)) return -1;
+  if (!format_file(qs, qidx, db, hits, nhits, alnbuf, alncap, o, out, m, json)) return -1;
   *text = to_c(out, len);
   return *text ? 0 : -1;
 }
 
 // AlnWriter::GetOutputFilename (cualn/AlnWriter.cpp:352-376):
-// <outdir>/<query name without a short extension>__<serial>.out
-extern "C" int cth_write_results_plain(const cth_pset* qs, int qidx, const cth_pset* db, const cth_hit* hits,
-                                       int nhits, const char* alnbuf, uint64_t alncap, const cth_outinfo* o,
-                                       const char* outdir, char* path, int pathcap) {
+// <outdir>/<query name without a short extension>__<serial>.out (.json)
+int write_results(const cth_pset* qs, int qidx, const cth_pset* db, const cth_hit* hits, int nhits,
+                  const char* alnbuf, uint64_t alncap, const cth_outinfo* o, const char* outdir, char* path,
+                  int pathcap, bool json) {
   std::string out;
   Meta m;
-  if (!format_file(qs, qidx, db, hits, nhits, alnbuf, alncap, o, out, m)) return -1;
+  if (!format_file(qs, qidx, db, hits, nhits, alnbuf, alncap, o, out, m, json)) return -1;
   std::string fn = outdir ? outdir : "";
   if (!fn.empty() && fn.back() != '/') fn += '/';
   std::string qn = base_name(m.qname);
   const size_t dot = qn.rfind('.');
   if (dot != std::string::npos && qn.size() - dot <= 4) qn = qn.substr(0, dot);
   char tail[32];
-  snprintf(tail, sizeof tail, "__%d.out", o->qrysernr);
+  snprintf(tail, sizeof tail, "__%d.%s", o->qrysernr, json ? "json" : "out");
   fn += qn + tail;
   FILE* fp = fopen(fn.c_str(), "wb");
   if (!fp) {
@@ -471,6 +687,39 @@ extern "C" int cth_write_results_plain(const cth_pset* qs, int qidx, const cth_p
   }
   if (path && pathcap > 0) snprintf(path, pathcap, "%s", fn.c_str());
   return 0;
+}
+
+}  // namespace
+
+extern "C" int cth_format_hit_plain(const cth_pset* qs, int qidx, const cth_pset* db, const cth_hit* hit,
+                                    const char* rows, const cth_outinfo* o, char** annot, char** rec) {
+  return format_hit(qs, qidx, db, hit, rows, o, annot, rec, false);
+}
+extern "C" int cth_format_hit_json(const cth_pset* qs, int qidx, const cth_pset* db, const cth_hit* hit,
+                                   const char* rows, const cth_outinfo* o, char** annot, char** rec) {
+  return format_hit(qs, qidx, db, hit, rows, o, annot, rec, true);
+}
+
+extern "C" int cth_format_results_plain(const cth_pset* qs, int qidx, const cth_pset* db, const cth_hit* hits,
+                                        int nhits, const char* alnbuf, uint64_t alncap, const cth_outinfo* o,
+                                        char** text, uint64_t* len) {
+  return format_results(qs, qidx, db, hits, nhits, alnbuf, alncap, o, text, len, false);
+}
+extern "C" int cth_format_results_json(const cth_pset* qs, int qidx, const cth_pset* db, const cth_hit* hits,
+                                       int nhits, const char* alnbuf, uint64_t alncap, const cth_outinfo* o,
+                                       char** text, uint64_t* len) {
+  return format_results(qs, qidx, db, hits, nhits, alnbuf, alncap, o, text, len, true);
+}
+
+extern "C" int cth_write_results_plain(const cth_pset* qs, int qidx, const cth_pset* db, const cth_hit* hits,
+                                       int nhits, const char* alnbuf, uint64_t alncap, const cth_outinfo* o,
+                                       const char* outdir, char* path, int pathcap) {
+  return write_results(qs, qidx, db, hits, nhits, alnbuf, alncap, o, outdir, path, pathcap, false);
+}
+extern "C" int cth_write_results_json(const cth_pset* qs, int qidx, const cth_pset* db, const cth_hit* hits,
+                                      int nhits, const char* alnbuf, uint64_t alncap, const cth_outinfo* o,
+                                      const char* outdir, char* path, int pathcap) {
+  return write_results(qs, qidx, db, hits, nhits, alnbuf, alncap, o, outdir, path, pathcap, true);
 }
 
 extern "C" void cth_free_text(char* p) { free(p); }

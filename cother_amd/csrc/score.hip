@@ -17,6 +17,8 @@
 // LDS.  The per-cell DDMS dynamic programme (the dominant cost, ~2200 cell
 // updates per score cell for full lists) runs entirely from LDS and
 // registers.
+#include <float.h>
+
 #include <hip/hip_runtime.h>
 
 #include "cth_dev.h"
@@ -30,7 +32,6 @@ constexpr int kListPitch = 132; // LDS bytes per template list
 
 struct ScoreLds {
   uint8_t tl[kCols * kListPitch];
-  int32_t qo[kRows][128];  // query lists: entry 0 = length, then 88 * value
   float dms[22 * 22];
   float ss[467];
   float cvvals[CTH_CVMAXCARD];
@@ -84,25 +85,59 @@ __device__ __forceinline__ uint32_t mkc(int qb, int qe, int pb, int pe) {
 // are exact pair scores and cells 1 and 3 their means; per 4-cell step the
 // best qualifying segment (length >= DSTSEGM) goes to the top-3 register.
 //
-// Branch-free form of the reference's per-cell update: a segment's begin
-// index is kept on one axis only (the other is that index plus the diagonal
-// offset) and relative to the current 4-cell step, so that the per-cell
-// indices are the literals 1..4; "score > 0 and better than the step's best" reduces to
-// "better than the step's best" because the best starts at 0; and the top-3
-// register is entered only when the step's best can displace its smallest
-// entry (the register stays ordered m0 >= m1 >= m2 >= 0).  The clamp at 0
-// is a max: it differs from the reference's select only in the sign of a zero
-// sum, and a zero is never registered.  The query list is
-// the same for the whole wavefront, so its values are read into scalar
-// registers.
-#define CTH_DDMS_UPD(K, SK, IDX)                                               \
+// Branch-free form of the reference's per-cell update.  A segment's begin
+// index is kept on the scanned axis only (the other is that index plus the
+// diagonal offset) and relative to the base of the current pair of 4-cell
+// steps, so the cell indices and length bounds are the literals 1..8.
+// "Score > 0 and better than the step's best" is "better than the step's
+// best" for cells 2-4 of a step because the best starts at 0, and for its
+// cell 1 the length test alone decides: a zero taken there is never
+// registered.  The clamp at 0 is a max (it differs from the reference's
+// select only in the sign of a zero sum, and a zero is never registered).
+// The mean of two pair scores enters as fmaf(0.5, a + b, segsco): 0.5 (a + b)
+// is exact, so the one rounding of the fused operation equals the
+// reference's rounding of segsco + 0.5f * (a + b).  The top-3 register is
+// entered only when the step's best can displace its smallest entry:
+// scom >= max(m2, FLT_MIN) (positive sums of the pair scores are far above
+// FLT_MIN, so this is scom > 0 && scom >= m2).
+#define CTH_DDMS_FIRST(K, SUM)                                                \
   {                                                                           \
     segr = (segsco != 0.0f) ? segr : (K);                                     \
-    segsco = fmaxf(segsco + (SK), 0.0f);                                      \
-    const bool take = (scom < segsco) & (segr <= (K) + 1 - CTH_DSTSEGM);      \
-    scom = take ? segsco : scom;                                              \
-    bb = take ? segr : bb;                                                    \
-    be = take ? (K) : be;                                                     \
+    segsco = fmaxf(SUM, 0.0f);                                                \
+    take1 = segr <= (K) + 1 - CTH_DSTSEGM;                                    \
+    scom = take1 ? segsco : 0.0f;                                             \
+    bb1 = segr;                                                               \
+  }
+#define CTH_DDMS_NEXT(K, SUM, TAKE, BB)                                       \
+  {                                                                           \
+    segr = (segsco != 0.0f) ? segr : (K);                                     \
+    segsco = fmaxf(SUM, 0.0f);                                                \
+    TAKE = (scom < segsco) & (segr <= (K) + 1 - CTH_DSTSEGM);                 \
+    scom = TAKE ? segsco : scom;                                              \
+    BB = segr;                                                                \
+  }
+// one 4-cell step, cells K0+1..K0+4 relative to the base ib, with pair
+// scores S0 (cell K0, the previous step's last cell), A2, A4 (cells K0+2 and
+// K0+4); the begin and end of the step's best are recovered from the take
+// flags only when it is registered
+#define CTH_DDMS_4CELL(K0, S0, A2, A4, QOFF, POFF)                            \
+  {                                                                           \
+    float scom;                                                               \
+    bool take1, take2, take3, take4;                                          \
+    int bb1, bb2, bb3, bb4;                                                   \
+    CTH_DDMS_FIRST((K0) + 1, fmaf(0.5f, S0 + A2, segsco))                     \
+    CTH_DDMS_NEXT((K0) + 2, segsco + A2, take2, bb2)                          \
+    CTH_DDMS_NEXT((K0) + 3, fmaf(0.5f, A2 + A4, segsco), take3, bb3)          \
+    CTH_DDMS_NEXT((K0) + 4, segsco + A4, take4, bb4)                          \
+    if (scom >= m2f) {                                                        \
+      int bb = take1 ? bb1 : 0, be = (K0) + 1;                                \
+      bb = take2 ? bb2 : bb; be = take2 ? (K0) + 2 : be;                      \
+      bb = take3 ? bb3 : bb; be = take3 ? (K0) + 3 : be;                      \
+      bb = take4 ? bb4 : bb; be = take4 ? (K0) + 4 : be;                      \
+      bb += ib; be += ib;                                                     \
+      register_max(m0, m1, m2, c0, c1, c2, scom, mkc(bb + (QOFF), be + (QOFF), bb + (POFF), be + (POFF))); \
+      m2f = fmaxf(m2, FLT_MIN);                                               \
+    }                                                                         \
   }
 
 // the pair score of list values whose table byte offsets are qb (query,
@@ -117,65 +152,52 @@ __device__ float ddms_cell(const float* __restrict__ dmsT, const int32_t* __rest
                            const uint8_t* __restrict__ t) {
   const int len1 = q[0], len2 = t[0];
   if (len1 < CTH_DSTSEGM || len2 < CTH_DSTSEGM) return 0.0f;
-  float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f;
+  float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f, m2f = FLT_MIN;
   uint32_t c0 = 0xffffffffu, c1 = 0xffffffffu, c2 = 0xffffffffu;
-  // upper part: diagonal j pairs query index i with template index j + i;
-  // i + segr is the query index where the running segment begins
+  // upper part: diagonal j pairs query index i with template index j + i
   for (int j = len2 >> 2; 0 <= j; j -= 2) {
     const uint8_t* tj = t + 1 + j;
-    int segr = 4;  // relative to the first step's index 0 after the step's "segr -= 4"
-    float s0 = pair(dmsT, q[1], tj[0]);
-    float segsco = s0 < 0.0f ? 0.0f : s0;
     const int iend = min(len1, len2 - j);
+    float s0 = pair(dmsT, q[1], tj[0]);
+    float segsco = fmaxf(s0, 0.0f);
+    int segr = 0;
     float s2 = pair(dmsT, q[1 + 2], tj[2]);
     float s4 = pair(dmsT, q[1 + 4], tj[4]);
-    for (int i = 0; i < iend; i += 4) {
-      // the next step's pair scores, loaded one step ahead (lists are padded)
-      const float n2 = pair(dmsT, q[1 + i + 6], tj[i + 6]);
-      const float n4 = pair(dmsT, q[1 + i + 8], tj[i + 8]);
-      const float s3 = 0.5f * (s2 + s4);
-      const float s1 = 0.5f * (s0 + s2);
-      float scom = 0.0f;
-      int bb = 0, be = 0;
-      segr -= 4;
-      CTH_DDMS_UPD(1, s1, i)
-      CTH_DDMS_UPD(2, s2, i)
-      CTH_DDMS_UPD(3, s3, i)
-      CTH_DDMS_UPD(4, s4, i)
-      if (scom != 0.0f && m2 <= scom)
-        register_max(m0, m1, m2, c0, c1, c2, scom, mkc(bb + i, be + i, bb + i + j, be + i + j));
-      s0 = s4;
-      s2 = n2;
-      s4 = n4;
+    for (int ib = 0; ib < iend; ib += 8) {
+      // two steps per iteration; the pair scores of the step after are loaded
+      // one step ahead (lists are padded)
+      const float n2 = pair(dmsT, q[1 + ib + 6], tj[ib + 6]);
+      const float n4 = pair(dmsT, q[1 + ib + 8], tj[ib + 8]);
+      CTH_DDMS_4CELL(0, s0, s2, s4, 0, j)
+      const float p2 = pair(dmsT, q[1 + ib + 10], tj[ib + 10]);
+      const float p4 = pair(dmsT, q[1 + ib + 12], tj[ib + 12]);
+      if (ib + 4 < iend) CTH_DDMS_4CELL(4, s4, n2, n4, 0, j)
+      segr -= 8;
+      s0 = n4;
+      s2 = p2;
+      s4 = p4;
     }
   }
-  // lower part: diagonal i pairs query index i + j with template index j;
-  // j + segr is the template index where the running segment begins
+  // lower part: diagonal i pairs query index i + j with template index j
   for (int i = len1 >> 2; 0 < i; i -= 2) {
     const int32_t* qi = q + 1 + i;
-    int segr = 4;  // relative to the first step's index 0 after the step's "segr -= 4"
-    float s0 = pair(dmsT, qi[0], t[1]);
-    float segsco = s0 < 0.0f ? 0.0f : s0;
     const int jend = min(len2, len1 - i);
+    float s0 = pair(dmsT, qi[0], t[1]);
+    float segsco = fmaxf(s0, 0.0f);
+    int segr = 0;
     float s2 = pair(dmsT, qi[2], t[1 + 2]);
     float s4 = pair(dmsT, qi[4], t[1 + 4]);
-    for (int j = 0; j < jend; j += 4) {
-      const float n2 = pair(dmsT, qi[j + 6], t[1 + j + 6]);
-      const float n4 = pair(dmsT, qi[j + 8], t[1 + j + 8]);
-      const float s3 = 0.5f * (s2 + s4);
-      const float s1 = 0.5f * (s0 + s2);
-      float scom = 0.0f;
-      int bb = 0, be = 0;
-      segr -= 4;
-      CTH_DDMS_UPD(1, s1, j)
-      CTH_DDMS_UPD(2, s2, j)
-      CTH_DDMS_UPD(3, s3, j)
-      CTH_DDMS_UPD(4, s4, j)
-      if (scom != 0.0f && m2 <= scom)
-        register_max(m0, m1, m2, c0, c1, c2, scom, mkc(bb + j + i, be + j + i, bb + j, be + j));
-      s0 = s4;
-      s2 = n2;
-      s4 = n4;
+    for (int ib = 0; ib < jend; ib += 8) {
+      const float n2 = pair(dmsT, qi[ib + 6], t[1 + ib + 6]);
+      const float n4 = pair(dmsT, qi[ib + 8], t[1 + ib + 8]);
+      CTH_DDMS_4CELL(0, s0, s2, s4, i, 0)
+      const float p2 = pair(dmsT, qi[ib + 10], t[1 + ib + 10]);
+      const float p4 = pair(dmsT, qi[ib + 12], t[1 + ib + 12]);
+      if (ib + 4 < jend) CTH_DDMS_4CELL(4, s4, n2, n4, i, 0)
+      segr -= 8;
+      s0 = n4;
+      s2 = p2;
+      s4 = p4;
     }
   }
   float s = 0.0f;
@@ -185,11 +207,17 @@ __device__ float ddms_cell(const float* __restrict__ dmsT, const int32_t* __rest
   s = fmaf(CTH_DDMS_SLOPE * CTH_DDMS_SCALE, s, CTH_DDMS_ICEPT);
   return s * CTH_DDMSWGT;
 }
-#undef CTH_DDMS_UPD
+#undef CTH_DDMS_FIRST
+#undef CTH_DDMS_NEXT
+#undef CTH_DDMS_4CELL
 
+// kPreInit: the log-odds term is already in scores (the reduced-precision
+// contraction of init_mfma.hip); otherwise it is computed here in FP32.
+template <bool kPreInit>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_score(cth_dpset q, int qp, cth_dpset db, int64_t col0, int ncols,
                                                int64_t ldsc, const cth_tables* __restrict__ tabs,
-                                               float* __restrict__ scores, float* __restrict__ mods) {
+                                               float* __restrict__ scores, float* __restrict__ mods,
+                                               const int32_t* __restrict__ qoff) {
   __shared__ ScoreLds L;
   const int tid = threadIdx.x;
   const int lane = tid & 63, w = tid >> 6;
@@ -202,7 +230,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
   for (int k = tid; k < 22 * 22; k += 256) L.dms[k] = tabs->dms[k];
   for (int k = tid; k < 467; k += 256) L.ss[k] = tabs->ss[k];
   for (int k = tid; k < tabs->cvcard; k += 256) L.cvvals[k] = tabs->cvvals[k];
-  // stage 64 template lists (32 dwords each) and 4 query lists
+  // stage 64 template lists (32 dwords each)
   for (int k = tid; k < kCols * 32; k += 256) {
     const int c = k >> 5, d = k & 31;
     const int cc = blockIdx.x * kCols + c;
@@ -210,13 +238,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
     if (cc < ncols) v = reinterpret_cast<const uint32_t*>(db.dd + (size_t)(col0 + cc) * CTH_DDSTRIDE)[d];
     if (cc >= ncols && d == 0) v = 0x15151500u;  // empty list
     *reinterpret_cast<uint32_t*>(&L.tl[c * kListPitch + d * 4]) = v;
-  }
-  for (int k = tid; k < kRows * 128; k += 256) {
-    const int r = k >> 7, d = k & 127;
-    const int rr = blockIdx.y * kRows + r;
-    int v = (d == 0) ? 0 : CTH_DD_MISSING;
-    if (rr < lq) v = q.dd[(size_t)(q0 + rr) * CTH_DDSTRIDE + d];
-    L.qo[r][d] = (d == 0) ? v : v * 22 * (int)sizeof(float);
   }
   __syncthreads();
   if (row >= lq || col >= ncols) return;
@@ -227,15 +248,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
 
   // Init: log-odds (CuBatchScoreMatrix_Init.cu:129-144,173-185)
   float s = 0.0f;
-#pragma unroll
-  for (int a = 0; a < CTH_NAA; a++) {
-    const float f1 = q.trg[a * q.ld + qi];
-    const float f2 = q.bkg[qp * CTH_NAA + a];
-    const float num = f1 * db.trg[a * db.ld + tj];
-    const float den = 0.5f * (f2 + db.bkg[tp * CTH_NAA + a]);
-    s = s + num / den;
+  if constexpr (kPreInit) {
+    s = scores[row * ldsc + col];
+  } else {
+  #pragma unroll
+    for (int a = 0; a < CTH_NAA; a++) {
+      const float f1 = q.trg[a * q.ld + qi];
+      const float f2 = q.bkg[qp * CTH_NAA + a];
+      const float num = f1 * db.trg[a * db.ld + tj];
+      const float den = 0.5f * (f2 + db.bkg[tp * CTH_NAA + a]);
+      s = s + num / den;
+    }
+    s = cth_logf(s) - CTH_INITSHIFT;
   }
-  s = cth_logf(s) - CTH_INITSHIFT;
 
   // SS (CuBatchScoreMatrix_ssss.cuh:64-116)
   float sss = 0.0f;
@@ -280,7 +305,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
   s = s + sc;
   float ms = fmaf(-CTH_CVSSHIFT, CTH_CVSWGT, sc);
 
-  const float d = ddms_cell(L.dms, L.qo[w], &L.tl[lane * kListPitch]);
+  // the wavefront's query list (one row per wavefront) by scalar loads
+  const int urow = __builtin_amdgcn_readfirstlane(row);
+  const float d = ddms_cell(L.dms, qoff + (size_t)(q0 + urow) * CTH_DDSTRIDE, &L.tl[lane * kListPitch]);
   if (d != 0.0f) {
     s = s + d;
     ms = ms + fmaf(-CTH_DDMSSHIFT, CTH_DDMSWGT, d);
@@ -293,13 +320,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
 
 // Launch the score stage for query profile qp against template positions
 // [col0, col0+ncols) of db; outputs are [Lq][ldsc].
-extern "C" int cth_launch_score(const cth_dpset* q, int qp, const cth_dpset* db, int64_t col0, int ncols,
-                                int64_t ldsc, const cth_tables* dtabs, int lq, float* scores, float* mods,
-                                hipStream_t st) {
+// contraction 0: the FP32 log-odds term of the reference; 1 / 2: the term
+// from the BF16 / FP16 matrix-core product (init_mfma.hip) first.
+extern "C" int cth_launch_init_mfma(const cth_dpset* q, int qp, const cth_dpset* db, int64_t col0, int ncols,
+                                    int64_t ldsc, int lq, int prec, float* out, hipStream_t st);
+extern "C" int cth_launch_score(const cth_dpset* q, const int32_t* qoff, int qp, const cth_dpset* db, int64_t col0,
+                                int ncols, int64_t ldsc, const cth_tables* dtabs, int lq, float* scores, float* mods,
+                                int contraction, hipStream_t st) {
   if (ncols <= 0 || lq <= 0) return 0;
+  if (!qoff) { cth_set_error("score: query distance offsets not built"); return -1; }
   if (ldsc < ncols) { cth_set_error("score: ldsc %lld < ncols %d", (long long)ldsc, ncols); return -1; }
   dim3 grid((ncols + kCols - 1) / kCols, (lq + kRows - 1) / kRows);
-  hipLaunchKernelGGL(k_score, grid, dim3(256), 0, st, *q, qp, *db, col0, ncols, ldsc, dtabs, scores, mods);
+  if (contraction) {
+    if (cth_launch_init_mfma(q, qp, db, col0, ncols, ldsc, lq, contraction, scores, st)) return -1;
+    hipLaunchKernelGGL(k_score<true>, grid, dim3(256), 0, st, *q, qp, *db, col0, ncols, ldsc, dtabs, scores, mods, qoff);
+  } else {
+    hipLaunchKernelGGL(k_score<false>, grid, dim3(256), 0, st, *q, qp, *db, col0, ncols, ldsc, dtabs, scores, mods,
+                       qoff);
+  }
   CTH_HIP_CHECK(hipGetLastError());
   return 0;
 }

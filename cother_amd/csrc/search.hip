@@ -30,8 +30,8 @@
 #include "cth_search.h"
 #include "cth_stats.h"
 
-extern "C" int cth_launch_score(const cth_dpset* q, int qp, const cth_dpset* db, int64_t col0, int ncols,
-                                int64_t ldsc, const cth_tables* dtabs, int lq, float* scores, float* mods,
+extern "C" int cth_launch_score(const cth_dpset* q, const int32_t* qoff, int qp, const cth_dpset* db, int64_t col0, int ncols,
+                                int64_t ldsc, const cth_tables* dtabs, int lq, float* scores, float* mods, int contraction,
                                 hipStream_t st);
 extern "C" int cth_launch_dp_list(const cth_dpset* q, int qp, const cth_dpset* db, int p0, const int* plist, int np,
                                   int ltmax, int64_t col0, int64_t ldsc, const float* scores, uint8_t* btck,
@@ -116,6 +116,7 @@ extern "C" int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt
                           int* nhits, cth_search_stats* stats) {
   if (!c->has_q || !c->has_db) { cth_set_error("search: queries or database not uploaded"); return -1; }
   if (q0 < 0 || q1 > c->q.d.nprof || q0 >= q1) { cth_set_error("search: bad query range [%d,%d)", q0, q1); return -1; }
+  if (opt->contraction < 0 || opt->contraction > 2) { cth_set_error("search: contraction %d", opt->contraction); return -1; }
   CTH_HIP_CHECK(hipSetDevice(c->device));
   const cth_devset& db = c->db;
   const int nprof = db.d.nprof;
@@ -153,7 +154,7 @@ extern "C" int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt
       if (cth_ctx_reserve(c, (void**)&c->dpout, &c->cap_dpout, sizeof(cth_dpres) * (size_t)(p1 - p0))) return -1;
       const double t0 = now_ms();
       if (timing) CTH_HIP_CHECK(hipEventRecord(c->ev[0], c->st));
-      if (cth_launch_score(&c->q.d, qi, &db.d, col0, (int)ncols, ldsc, c->dtab, lq, c->scores, c->mods, c->st))
+      if (cth_launch_score(&c->q.d, c->q.ddoff, qi, &db.d, col0, (int)ncols, ldsc, c->dtab, lq, c->scores, c->mods, opt->contraction, c->st))
         return -1;
       if (timing) CTH_HIP_CHECK(hipEventRecord(c->ev[1], c->st));
       // phase 1 in two passes: the DP of every profile without traceback,

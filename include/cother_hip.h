@@ -82,6 +82,13 @@ typedef struct cth_ctx cth_ctx;
 int cth_ctx_create(int device, const char* envdir, cth_ctx** out);
 void cth_ctx_destroy(cth_ctx* c);
 const char* cth_last_error(void);
+/* A context with its own stream and scratch buffers that reads parent's
+ * tables, uploaded queries and resident database (one device-memory area of
+ * the reference's CuDeviceMemory, src/libmycu/cupro/CuDeviceMemory.cuh:
+ * 148-154).  Uploads to it are refused; the parent must outlive it. */
+int cth_ctx_create_shared(cth_ctx* parent, cth_ctx** out);
+/* Sizes of the uploaded sets (qlens: the context's host copy). */
+int cth_ctx_shape(const cth_ctx* c, int* nqueries, const int32_t** qlens, int* ndb, int64_t* dbpos);
 /* Number of visible HIP devices (0 when none; the reference's
  * Devices::RegisterDevices, src/libmycu/cualn/Devices.cu:59-108). */
 int cth_device_count(void);
@@ -100,6 +107,12 @@ int cth_upload_db(cth_ctx* c, const cth_pset* db);
  * (src/libmycu/cusco/CuBatchScoreMatrix.cu:122-404) computes on the device;
  * this entry exposes its outputs (scores, modscores) for inspection. */
 int cth_score_block(cth_ctx* c, int qidx, int p0, int p1, float* scores, float* modscores);
+/* As cth_score_block with the log-odds term from the reduced-precision
+ * matrix-core product (contraction 1 BF16, 2 FP16; 0 is cth_score_block). */
+int cth_score_block_ex(cth_ctx* c, int qidx, int p0, int p1, int contraction, float* scores, float* modscores);
+/* Device check of the matrix-core lane layout the contraction kernel
+ * assumes; *bad = number of mismatching accumulator values (0 expected). */
+int cth_selftest_mfma(int* bad);
 
 /* Phase 1 of query qidx against template profiles [p0,p1): score matrix,
  * DP with traceback (CuBatchDP::PerformCompleteDynProgDevice,
@@ -145,7 +158,8 @@ typedef struct cth_search_opts {
   int timing;         /* record HIP events around each stage on the context's stream */
   uint64_t mem_budget;/* bytes for the per-chunk score/traceback buffers (0: 16 GiB) */
   float minpp;        /* MAP posterior threshold (MINPP; 0: the default 0.28) */
-  int reserved;
+  int contraction;    /* log-odds term: 0 FP32 as the reference (default); 1 BF16, 2 FP16 operands on
+                         the matrix cores with FP32 accumulation (no reference counterpart) */
   char* alnbuf;       /* optional: receives the alignment rows of every hit (see cth_hit.alnoff) */
   uint64_t alncap;    /* capacity of alnbuf in bytes */
 } cth_search_opts;
@@ -196,6 +210,45 @@ int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt, cth_hit* 
  * AbstractScoreMatrix.cpp:246-395: GetSearchSpace, GetDeltaLength). */
 void cth_search_space(uint64_t qlen, uint64_t dbsize, uint64_t ndb, uint64_t* sspace, uint64_t* deltalen);
 
+/* ---- serving loop -----------------------------------------------------------
+ * Requests (one query of the uploaded set each) against the resident
+ * database, run concurrently on nslots slots (own stream and scratch per
+ * slot) and admitted by buffer footprint: a waiting request starts when a
+ * slot is free and the footprints in flight plus its own (Lq x chunk columns
+ * x 13 bytes with MAP, 9 without) fit mem_budget; the first waiting request
+ * that fits goes first.  Replaces the reference's batch dispatch
+ * (JobDispatcher::Run and the DevCommThread workers, src/libmycu/cualn/
+ * JobDispatcher.cpp:967-1157, DevCommThread.cpp:149-356) for a resident
+ * database.  A request's hits are those cth_search gives for that query. */
+typedef struct cth_server cth_server;
+typedef struct cth_server_opts {
+  int nslots;              /* requests on the device at once */
+  uint64_t mem_budget;     /* device bytes for the buffers of the requests in flight */
+  uint64_t chunk_cols;     /* database columns per chunk of a request (0: the whole database) */
+  cth_search_opts search;  /* per-request options (mem_budget, alnbuf, alncap are set per request) */
+  uint64_t alncap;         /* per-request alignment-row capacity (0: no rows) */
+} cth_server_opts;
+typedef struct cth_request_info {
+  int32_t qidx, slot;
+  uint64_t footprint;      /* admitted buffer bytes */
+  double ms_queued;        /* submit -> start */
+  double ms_latency;       /* submit -> done */
+  double ms_service;       /* start -> done */
+  cth_search_stats stats;
+} cth_request_info;
+void cth_server_opts_init(cth_server_opts* o);
+/* The parent context holds the uploaded queries and database; create the
+ * server after the uploads and destroy it before the parent.  Waiting on
+ * every submitted ticket before cth_server_destroy is the caller's part. */
+int cth_server_create(cth_ctx* parent, const cth_server_opts* o, cth_server** out);
+int cth_server_footprint(const cth_server* s, int qidx, uint64_t* bytes);
+int cth_server_submit(cth_server* s, int qidx, uint64_t* ticket);
+/* Block until the request is done; hits as cth_search returns them, with
+ * alnoff relative to alnbuf when rows were requested. */
+int cth_server_wait(cth_server* s, uint64_t ticket, cth_hit* hits, int maxhits, int* nhits, char* alnbuf,
+                    uint64_t alncap, cth_request_info* info);
+void cth_server_destroy(cth_server* s);
+
 /* ---- results file ----------------------------------------------------------
  * The reference's plain-text output for one query: the per-hit records of
  * CuBatchProcessingFinalizer::CompressResultsPlain (src/libmycu/cupro/
@@ -239,6 +292,17 @@ int cth_format_results_plain(const cth_pset* qs, int qidx, const cth_pset* db, c
 int cth_write_results_plain(const cth_pset* qs, int qidx, const cth_pset* db, const cth_hit* hits, int nhits,
                             const char* alnbuf, uint64_t alncap, const cth_outinfo* o, const char* outdir, char* path,
                             int pathcap);
+/* The same three in the reference's JSON format (CompressResultsJSON,
+ * src/libmycu/cupro/CuBatchProcessingFinalizerJSON.cpp:35-383;
+ * AlnWriter::WriteResultsJSON, src/libmycu/cualn/AlnWriterJSON.cpp:19-429);
+ * the file is <outdir>/<query name>__<qrysernr>.json. */
+int cth_format_hit_json(const cth_pset* qs, int qidx, const cth_pset* db, const cth_hit* hit, const char* rows,
+                        const cth_outinfo* o, char** annot, char** record);
+int cth_format_results_json(const cth_pset* qs, int qidx, const cth_pset* db, const cth_hit* hits, int nhits,
+                            const char* alnbuf, uint64_t alncap, const cth_outinfo* o, char** text, uint64_t* len);
+int cth_write_results_json(const cth_pset* qs, int qidx, const cth_pset* db, const cth_hit* hits, int nhits,
+                           const char* alnbuf, uint64_t alncap, const cth_outinfo* o, const char* outdir, char* path,
+                           int pathcap);
 void cth_free_text(char* p);
 
 /* Measurement aid (no reference counterpart): a plain device-to-device copy

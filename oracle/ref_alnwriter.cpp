@@ -10,7 +10,8 @@
 // them, :276-292), triggers the write as the job dispatcher does, and lets the
 // reference write the output file for query 0.
 //
-// usage: alnwriter_ref <options file> <cother.par> <outdir> <input file>
+// usage: alnwriter_ref [--json] <options file> <cother.par> <outdir> <input file>
+//        (--json: the JSON format, CLOptions B_FMT = ofJSON)
 //    or: alnwriter_ref --fmt <maxlen> <indent> <width> <lead char code> < text
 //        (the reference's FormatDescription, liblib/fmtdescription.h:35-83,
 //        applied to the text on stdin, result on stdout)
@@ -35,6 +36,7 @@
 #include "liblib/fmtdescription.h"
 #include "liblib/msg.h"
 #include "libpro/srcpro/Configuration.h"
+#include "libpro/srcpro/CLOptions.h"
 #include "libpro/srcpro/MOptions.h"
 #include "libmycu/cualn/AlnWriter.h"
 
@@ -76,6 +78,11 @@ int fmt_main(int argc, char** argv) {
 
 int main(int argc, char** argv) {
   if (argc > 1 && !strcmp(argv[1], "--fmt")) return fmt_main(argc, argv);
+  if (argc > 1 && !strcmp(argv[1], "--json")) {
+    CLOptions::AssignCLOptB_FMT(CLOptions::ofJSON);
+    argc--;
+    argv++;
+  }
   if (argc != 5) {
     fprintf(stderr, "usage: %s <options> <cother.par> <outdir> <input>\n", argv[0]);
     return 2;

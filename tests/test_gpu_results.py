@@ -1,4 +1,4 @@
-"""End to end: the plain-text results file of a GPU search equals, byte for
+"""End to end: the results files (plain text and JSON) of a GPU search equal, byte for
 byte, the file made from the oracle's hit records and the oracle's MAP
 alignment rows (the writer itself is pinned by the reference's AlnWriter in
 test_results_plain.py)."""
@@ -39,6 +39,7 @@ def test_results_file_matches_oracle(ctx, oracle, tmp_path):
         off += nh[qi]
         gh, gbuf = _buf(gh, ga)
         got = cother_amd.format_results_plain(q, qi, db, gh, gbuf, qrysernr=qi, **info)
+        got_json = cother_amd.format_results_plain(q, qi, db, gh, gbuf, fmt="json", qrysernr=qi, **info)
         want_hits = oracle.search_query(q, qi, db, eval_=100.0)
         sc, md = oracle.score_matrix(q, qi, db, 0, db.nprof)
         rows = []
@@ -49,6 +50,7 @@ def test_results_file_matches_oracle(ctx, oracle, tmp_path):
         wh, wbuf = _buf(want_hits, rows)
         want = cother_amd.format_results_plain(q, qi, db, wh, wbuf, qrysernr=qi, **info)
         assert got == want, f"query {qi}"
+        assert got_json == cother_amd.format_results_plain(q, qi, db, wh, wbuf, fmt="json", qrysernr=qi, **info)
         assert got.count(b"\n>") == len(want_hits)
         path = cother_amd.write_results_plain(q, qi, db, gh, gbuf, str(tmp_path), qrysernr=qi, **info)
         assert open(path, "rb").read() == got

@@ -1,0 +1,68 @@
+"""Serving loop (cth_server_*): requests of mixed lengths admitted by buffer
+footprint and run concurrently on several slots give, query by query, the
+same hits and alignment rows as cth_search alone."""
+import numpy as np
+import pytest
+
+import cother_amd
+
+pytestmark = pytest.mark.gpu
+
+
+def _alone(ctx, qi):
+    hits, nh, _, alns = ctx.search(qi, qi + 1, eval=10.0, alignments=True)
+    return hits, alns
+
+
+@pytest.fixture(scope="module")
+def sets():
+    # queries of mixed lengths; the first templates are copies of two queries
+    q = cother_amd.synth(9, 0, lmin=20, lmax=400, dist_mode=1, sort_desc=0, seed=131)
+    same = cother_amd.subset_list(q, [1, 4])
+    other = cother_amd.synth(300, 1, 220.0, 0.35, 30, 1500, dist_mode=1, sort_desc=1, seed=132)
+    return q, cother_amd.concat([same, other])
+
+
+@pytest.mark.parametrize("nslots,budget_cells", [(1, 0), (3, 0), (4, 2_000_000)])
+def test_results_alone_and_in_the_mix(ctx, sets, nslots, budget_cells):
+    q, db = sets
+    ctx.upload_queries(q)
+    ctx.upload_db(db)
+    print(f"serve: {q.nprof} queries x {db.nprof} templates, {nslots} slots", flush=True)
+    want = [_alone(ctx, qi) for qi in range(q.nprof)]
+    lens = q.lens()
+    # budget_cells 0: room for everything; otherwise a budget that holds only
+    # a few requests at once (admission must wait for footprints to free up)
+    chunk = 1 << 16
+    budget = int(budget_cells * 13) if budget_cells else 1 << 34
+    budget = max(budget, int(lens.max()) * chunk * 13)
+    srv = cother_amd.Server(ctx, nslots=nslots, mem_budget=budget, chunk_cols=chunk, alncap=1 << 22)
+    try:
+        order = list(range(q.nprof)) + [3, 0, 8]  # some queries twice
+        tickets = [(qi, srv.submit(qi)) for qi in order]
+        seen_slots = set()
+        for qi, t in tickets:
+            hits, alns, info = srv.wait(t)
+            assert info["qidx"] == qi
+            seen_slots.add(info["slot"])
+            assert info["footprint"] == int(lens[qi]) * min(chunk, int(db.npos) + 64) * 13
+            wh, wa = want[qi]
+            assert len(hits) == len(wh), f"query {qi}"
+            for f in ("profile", "score", "logE", "beg", "end", "mapbeg", "mapend", "alnlen", "idnts", "psts"):
+                assert (hits[f].view(np.uint32) == wh[f].view(np.uint32)).all(), f"query {qi} field {f}"
+            assert alns == wa, f"query {qi}"
+        print(f"slots used: {sorted(seen_slots)}", flush=True)
+    finally:
+        srv.close()
+
+
+def test_request_over_budget_is_refused(ctx, sets):
+    q, db = sets
+    ctx.upload_queries(q)
+    ctx.upload_db(db)
+    srv = cother_amd.Server(ctx, nslots=1, mem_budget=1 << 20, chunk_cols=1 << 16)
+    try:
+        with pytest.raises(RuntimeError, match="budget"):
+            srv.submit(0)
+    finally:
+        srv.close()

@@ -1,12 +1,12 @@
-"""The plain-text results file (cth_format_hit_plain, cth_format_results_plain,
-cth_write_results_plain), pinned by the reference's own results writer.
+"""The results files, plain text and JSON (cth_format_hit_*, cth_format_results_*,
+cth_write_results_*), pinned by the reference's own results writer.
 
 - whole files: the reference's AlnWriter (cualn/AlnWriter.cpp,
   AlnWriterPlain.cpp), built from its sources into oracle/_ref/ by
   oracle/Makefile.ref, receives our per-hit records split into parts, merges
-  them and writes the file; our writer must produce the same bytes, for
-  empty and non-empty hit lists, NOHITS/NOALNS limits, long database names
-  and several parts;
+  them and writes the file (plain, or JSON with B_FMT set); our writer must
+  produce the same bytes, for empty and non-empty hit lists, NOHITS/NOALNS
+  limits, long database names and several parts;
 - descriptions: the annotation and record headers equal the reference's
   FormatDescription (liblib/fmtdescription.h) applied to the same text;
 - records: the formatting of the finalizer (cupro/CuBatchProcessingFinalizerPlain.cpp,
@@ -119,11 +119,11 @@ def sets():
     return qs, db
 
 
-def _ref_file(tmp_path, qs, db, hits, rows, info, nparts, options=OPTIONS):
+def _ref_file(tmp_path, qs, db, hits, rows, info, nparts, options=OPTIONS, fmt="plain"):
     """The reference writer's file for our per-hit records in nparts parts."""
     lq = int(qs.lens()[0])
     ss, dl = cother_amd.search_space(lq, info["dbsize"], info["ndbentries"])
-    recs = [cother_amd.format_hit_plain(qs, 0, db, h, r, **info) for h, r in zip(hits, rows)]
+    recs = [cother_amd.format_hit_plain(qs, 0, db, h, r, fmt=fmt, **info) for h, r in zip(hits, rows)]
     # the finalizer hands over only hits below the threshold (CompressResultsPlain)
     order = np.array([k for k in range(len(hits)) if not (np.float32(info["logevthld"]) < hits[k]["logE"])], int)
     parts = np.array_split(order, nparts) if len(order) else [order]
@@ -140,11 +140,12 @@ def _ref_file(tmp_path, qs, db, hits, rows, info, nparts, options=OPTIONS):
         for k in part:
             a, b = recs[k]
             inp += f"{float(hits[k]['logE']).hex()} {len(a)} {len(b)}\n".encode() + a + b
-    d = tmp_path / f"ref{nparts}"
+    d = tmp_path / f"ref{nparts}{fmt}"
     d.mkdir(exist_ok=True)
-    f = tmp_path / f"in{nparts}.txt"
+    f = tmp_path / f"in{nparts}{fmt}.txt"
     f.write_bytes(bytes(inp))
-    r = subprocess.run([WRITER, options, PARFILE, str(d), str(f)], capture_output=True, text=True)
+    flag = ["--json"] if fmt == "json" else []
+    r = subprocess.run([WRITER, *flag, options, PARFILE, str(d), str(f)], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout + r.stderr
     files = os.listdir(d)
     assert len(files) == 1, files
@@ -152,20 +153,25 @@ def _ref_file(tmp_path, qs, db, hits, rows, info, nparts, options=OPTIONS):
 
 
 @need_ref
+@pytest.mark.parametrize("fmt", ["plain", "json"])
 @pytest.mark.parametrize("nhits,nparts", [(0, 1), (1, 1), (7, 2), (25, 3)])
-def test_file_matches_reference_writer(tmp_path, sets, nhits, nparts):
+def test_file_matches_reference_writer(tmp_path, sets, nhits, nparts, fmt):
     qs, db = sets
     rng = np.random.default_rng(100 + nhits)
     hits, rows = _hits(rng, nhits, db, int(qs.lens()[0]))
     buf = _alnbuf(hits, rows)
     info = dict(dbnames=["/data/dbs/pdb70_2021", "scop"], dbsize=int(db.npos), ndbentries=db.nprof,
                 logevthld=math.log(10.0))
-    name, want = _ref_file(tmp_path, qs, db, hits, rows, info, nparts)
+    name, want = _ref_file(tmp_path, qs, db, hits, rows, info, nparts, fmt=fmt)
     ours = tmp_path / "ours"
     ours.mkdir()
-    path = cother_amd.write_results_plain(qs, 0, db, hits, buf, str(ours), **info)
+    path = cother_amd.write_results_plain(qs, 0, db, hits, buf, str(ours), fmt=fmt, **info)
     assert os.path.basename(path) == name
     got = open(path, "rb").read()
+    if fmt == "json":
+        import json
+        doc = json.loads(got)["cother_search"]  # valid JSON when NOALNS >= NOHITS
+        assert len(doc["search_hits"]) == len(doc["search_summary"])
     if got != want:
         a, b = got.decode(errors="replace").splitlines(), want.decode(errors="replace").splitlines()
         for i, (x, y) in enumerate(zip(a, b)):
@@ -175,7 +181,8 @@ def test_file_matches_reference_writer(tmp_path, sets, nhits, nparts):
 
 
 @need_ref
-def test_limits_and_long_names(tmp_path, sets):
+@pytest.mark.parametrize("fmt", ["plain", "json"])
+def test_limits_and_long_names(tmp_path, sets, fmt):
     """NOHITS/NOALNS below the number of hits, a long database name list."""
     qs, db = sets
     rng = np.random.default_rng(5)
@@ -191,8 +198,8 @@ def test_limits_and_long_names(tmp_path, sets):
     names = ["/x/" + "n" * 130] + [f"db_{k}_" + "q" * 40 for k in range(30)]
     info = dict(dbnames=names, dbsize=int(db.npos), ndbentries=db.nprof, logevthld=math.log(0.5),
                 nohits=5, noalns=3, alnwidth=37, dscwidth=50)
-    _, want = _ref_file(tmp_path, qs, db, hits, rows, info, 2, options=str(of))
-    got = cother_amd.format_results_plain(qs, 0, db, hits, buf, **info)
+    _, want = _ref_file(tmp_path, qs, db, hits, rows, info, 2, options=str(of), fmt=fmt)
+    got = cother_amd.format_results_plain(qs, 0, db, hits, buf, fmt=fmt, **info)
     assert got == want
 
 
