@@ -158,6 +158,37 @@ def serve_phase(ctx, qs, db, args, seq_qps):
             "mem_budget_gib": 64, "chunk_cols": 4 << 20}
 
 
+def long_profile_run(args, threads):
+    """One query of 2000 positions with a full predicted distance map against 5000
+    templates of 1000-3000 positions (C-alpha chain distances): long DP/MAP rows, a
+    database processed in several chunks; one JSON line of its own."""
+    import cother_amd
+    q = cother_amd.synth(1, 0, 0.0, 0.0, 2000, 2000, dist_mode=0, sort_desc=0, seed=args.seed + 11, threads=threads)
+    db = cother_amd.synth(5000, 0, 0.0, 0.0, 1000, 3000, dist_mode=1, sort_desc=1, seed=args.seed + 12,
+                          threads=threads)
+    ctx = cother_amd.Context(0)
+    ctx.upload_queries(q)
+    ctx.upload_db(db)
+    log(f"long profile: Lq={int(q.lens()[0])} vs {db.nprof} templates ({db.npos} positions)")
+    ctx.search(0, 1, eval=args.eval, contraction=args.contraction)  # warmup
+    t1 = time.perf_counter()
+    hits, nh, st = ctx.search(0, 1, eval=args.eval, timing=True, contraction=args.contraction)
+    wall = time.perf_counter() - t1
+    out = {"metric": METRIC, "value": round(st.cells / wall / 1e9, 4), "unit": "GCUPS", "n_gpus": 1, "steps": 1,
+           "warmup": 1, "ms_per_step": round(wall * 1e3, 2), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "long-profile stress: one query of 2000 positions (predicted distance map) vs "
+                                  "5000 templates of 1000-3000 positions",
+                      "db_positions": int(db.npos), "cells": st.cells, "chunks": st.nchunks,
+                      "contraction": ("fp32", "bf16-mfma", "fp16-mfma")[args.contraction]},
+           "stage_ms": {"score": round(st.ms_k_score, 1), "dp": round(st.ms_k_dp, 1), "stats": round(st.ms_k_stats, 1),
+                        "map": round(st.ms_k_map, 1)},
+           "hits": int(nh[0]), "passed": st.npassed, "map_cells": st.map_cells,
+           "longest_alignment": int(hits["alnlen"].max()) if len(hits) else 0}
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,9 +201,15 @@ def main():
     ap.add_argument("--seed", type=int, default=20260)
     ap.add_argument("--serve-requests", type=int, default=16,
                     help="serving phase: requests (the timed queries again) arriving as a Poisson process (0: skip)")
-    ap.add_argument("--serve-load", type=float, default=1.0,
-                    help="serving phase: offered load as a multiple of the sequential query rate")
+    ap.add_argument("--serve-load", type=float, default=1.5,
+                    help="serving phase: offered load as a multiple of the sequential query rate "
+                         "(above 1 the queue never drains, so the phase's GCUPS is the sustained rate)")
     ap.add_argument("--serve-slots", type=int, default=2)
+    ap.add_argument("--contraction", type=int, default=0, choices=(0, 1, 2),
+                    help="log-odds term: 0 FP32 (the reference's arithmetic), 1 BF16 / 2 FP16 matrix-core variant")
+    ap.add_argument("--long-profile", action="store_true",
+                    help="long-profile stress instead: one query of 2000 positions (predicted distance map) "
+                         "against 5000 templates of 1000-3000 positions")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel over BYTES (traffic-counter calibration) and exit")
     args = ap.parse_args()
@@ -192,6 +229,9 @@ def main():
                           "GBps": 2 * args.copy_calibration / (ms.value * 1e-3) / 1e9}), flush=True)
         return
     threads = min(16, os.cpu_count() or 1)
+    if args.long_profile:
+        long_profile_run(args, threads)
+        return
     t0 = time.perf_counter()
     qs = cother_amd.synth(args.queries, 1, 250.0, 0.35, 40, 1000, dist_mode=0, sort_desc=0, seed=args.seed,
                           threads=threads)
@@ -211,7 +251,7 @@ def main():
 
     for s in range(args.warmup):
         qi = s % qs.nprof
-        ctx.search(qi, qi + 1, eval=args.eval)
+        ctx.search(qi, qi + 1, eval=args.eval, contraction=args.contraction)
         log(f"warmup step {s} (query {qi}, Lq={ql[qi]}) done")
 
     lat, cells, ops, ms_score, nlaunch, ms_dp, ms_st, npass = [], 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0
@@ -220,7 +260,7 @@ def main():
     for s in range(args.steps):
         qi = (args.warmup + s) % qs.nprof
         t1 = time.perf_counter()
-        hits, nh, st = ctx.search(qi, qi + 1, eval=args.eval, timing=True)
+        hits, nh, st = ctx.search(qi, qi + 1, eval=args.eval, timing=True, contraction=args.contraction)
         lat.append((time.perf_counter() - t1) * 1e3)
         cells += st.cells
         ms_score += st.ms_k_score
@@ -263,7 +303,8 @@ def main():
                                "(lognormal median 220, 30-1500); one query per step",
                    "queries": qs.nprof, "templates": db.nprof, "db_positions": int(db.npos),
                    "query_positions_timed": int(sum(ql[(args.warmup + s) % qs.nprof] for s in range(args.steps))),
-                   "eval": args.eval, "map": True},
+                   "eval": args.eval, "map": True,
+                   "contraction": ("fp32", "bf16-mfma", "fp16-mfma")[args.contraction]},
         "p50_ms": round(float(np.percentile(lat_a, 50)), 2), "p99_ms": round(float(np.percentile(lat_a, 99)), 2),
         "stage_ms": {"score": round(ms_score, 1), "dp": round(ms_dp, 1), "stats": round(ms_st, 1),
                      "map": round(ms_map, 1)},

@@ -52,26 +52,35 @@ __device__ __forceinline__ bool coords_intersect(uint32_t c1, uint32_t c2) {
   return qe >= qb || pe >= pb;
 }
 
-// RegisterMaxDDMScore (CuBatchScoreMatrix_ddms2s.cuh:168-261)
+// RegisterMaxDDMScore (CuBatchScoreMatrix_ddms2s.cuh:168-261): keep the
+// three best segment scores m0 >= m1 >= m2 whose coordinates do not
+// intersect.  Called only for s >= max(m2, FLT_MIN).  The reference's three
+// cases, as selects on the two intersection tests with the current m0 and m1:
+//  s >= m0: s becomes m0; the old m0 stays (as m1) unless it intersects s,
+//           the old m1 stays (next) unless it intersects s, the old m2 drops;
+//  m1 <= s < m0, s not intersecting m0: s becomes m1, the old m1 becomes m2
+//           unless it intersects s;
+//  m2 <= s < m1, s intersecting neither m0 nor m1: s becomes m2.
+// An entry whose value is 0 may keep stale coordinates: every intersection
+// test with it only decides between two zeros, and coordinates of list
+// indices (< 99) never intersect the initial 0xffffffff.
 __device__ __forceinline__ void register_max(float& m0, float& m1, float& m2, uint32_t& c0, uint32_t& c1,
                                              uint32_t& c2, float s, uint32_t sc) {
-  if (!s) return;
-  if (m0 <= s) {
-    m2 = m1; c2 = c1;
-    m1 = m0; c1 = c0;
-    m0 = s; c0 = sc;
-    if (m1 != 0.0f && coords_intersect(c0, c1)) m1 = 0.0f;
-    if (m2 != 0.0f && coords_intersect(c0, c2)) m2 = 0.0f;
-    if (m1 == 0.0f) { m1 = m2; c1 = c2; m2 = 0.0f; }
-  } else if (m1 <= s) {
-    if (coords_intersect(sc, c0)) return;
-    m2 = m1; c2 = c1;
-    m1 = s; c1 = sc;
-    if (coords_intersect(c1, c2)) m2 = 0.0f;
-  } else if (m2 <= s) {
-    if (coords_intersect(sc, c0) || coords_intersect(sc, c1)) return;
-    m2 = s; c2 = sc;
-  }
+  const bool i0 = coords_intersect(sc, c0), i1 = coords_intersect(sc, c1);
+  const bool a = m0 <= s, b1 = m1 <= s;
+  const bool b = !a & b1 & !i0;
+  const bool c = !a & !b1 & !i0 & !i1;
+  const float ym1 = i1 ? 0.0f : m1;
+  const float n1 = a ? (i0 ? ym1 : m0) : (b ? s : m1);
+  const uint32_t d1 = a ? (i0 ? c1 : c0) : (b ? sc : c1);
+  const float n2 = a ? (i0 ? 0.0f : ym1) : (b ? ym1 : (c ? s : m2));
+  const uint32_t d2 = (a | b) ? c1 : (c ? sc : c2);
+  m0 = a ? s : m0;
+  c0 = a ? sc : c0;
+  m1 = n1;
+  c1 = d1;
+  m2 = n2;
+  c2 = d2;
 }
 
 __device__ __forceinline__ uint32_t mkc(int qb, int qe, int pb, int pe) {

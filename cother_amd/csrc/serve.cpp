@@ -16,9 +16,11 @@
 //    queue is scanned in arrival order and the first request that fits is
 //    admitted, so a long query that does not fit yet does not hold back
 //    shorter ones behind it;
-//  - a slot returns scratch larger than its next request needs before
-//    running it, so the device memory held by the slots stays near the
-//    footprints in flight.
+//  - a slot keeps its scratch between requests (freeing device memory
+//    synchronizes the device, which would stall the other slots), except
+//    that scratch beyond both its share of the budget (mem_budget / nslots)
+//    and its next request's footprint is returned first, so the slots hold
+//    at most about max(mem_budget, one footprint per slot).
 //
 // Results are those of cth_search for the same query: requests do not
 // share buffers, and every kernel's output depends only on its own inputs.
@@ -115,7 +117,7 @@ struct cth_server {
         r->slot = slot;
         r->t_start = now_ms();
       }
-      cth_ctx_trim(c, r->footprint);
+      cth_ctx_trim(c, std::max<uint64_t>(r->footprint, opt.mem_budget / slots.size()));
       cth_search_opts so = opt.search;
       so.mem_budget = r->footprint;
       so.alnbuf = nullptr;
