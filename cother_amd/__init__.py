@@ -144,6 +144,10 @@ def lib():
     L.cth_write_results_json.argtypes = L.cth_write_results_plain.argtypes
     L.cth_free_text.argtypes = [ctypes.c_void_p]
     L.cth_free_text.restype = None
+    L.cth_read_distances.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                                     ctypes.c_float, ctypes.c_void_p]
+    L.cth_add_distances.argtypes = [P, ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                                    ctypes.c_float]
     L.cth_ctx_create_shared.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]
     L.cth_ctx_shape.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_void_p,
                                 ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int64)]
@@ -328,6 +332,20 @@ def pset_desc(ps, k):
     v = lib().cth_pset_desc(ps.ptr, k)
     return None if v is None else v.decode()
 
+
+def read_distances(path, prolen, columns=(3,), prbthrhld=0.0):
+    """Per-position distance lists (float32 arrays) of a distance file, as adddist compiles them."""
+    cols = (ctypes.c_int * max(1, len(columns)))(*columns)
+    out = np.zeros((max(1, prolen), 100), np.float32)
+    _check(lib().cth_read_distances(path.encode(), prolen, cols, len(columns), prbthrhld, out.ctypes.data),
+           "cth_read_distances")
+    return [out[m, 1:1 + int(out[m, 0])].copy() for m in range(prolen)]
+
+
+def add_distances(ps, k, path, columns=(3,), prbthrhld=0.0):
+    """Replace profile k's distance lists by those of a distance file."""
+    cols = (ctypes.c_int * max(1, len(columns)))(*columns)
+    _check(lib().cth_add_distances(ps.ptr, k, path.encode(), cols, len(columns), prbthrhld), "cth_add_distances")
 
 def selftest_mfma():
     """Number of mismatching accumulators of the MFMA lane-layout check (0 expected)."""

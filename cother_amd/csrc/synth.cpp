@@ -33,6 +33,7 @@
 #include <utility>
 #include <vector>
 
+#include "cth_adddist.h"
 #include "cth_pset.h"
 
 namespace {
@@ -45,9 +46,6 @@ const float kRobinson[CTH_NAA] = {
 const char kAA[] = "ARNDCQEGHILKMFPSTWYV";  // liblib/alpha.h:31
 const char kSS[] = " eh";                   // SS state letters as the device sees them (liblib/alpha.cpp:14)
 
-// adddist options (env/options.txt DSTSEGM/DSTNPOS/DSTSEP6/DSTGAPL/DSTFREE)
-const int kDstSegm = 6, kDstNpos = 12, kDstFree = 3;
-const float kDstSep6 = 12.f, kDstGapl = 24.f;
 
 struct Rng {
   uint64_t s;
@@ -118,49 +116,6 @@ void sample_transitions(Rng& r, float out7[CTH_NTRN]) {
   }
 }
 
-// restatement of adddist's ProcessDistancesForPosition (adddist.cpp:560-651)
-void process_position(int position, int L, std::vector<float>& vdsts, std::vector<int>& vposs,
-                      std::vector<std::vector<std::pair<float, int>>>& tri,
-                      std::vector<std::vector<float>>& dgram) {
-  for (const auto& dn : tri[position]) { vdsts.push_back(dn.first); vposs.push_back(dn.second); }
-  std::vector<int> ndx(vdsts.size());
-  std::iota(ndx.begin(), ndx.end(), 0);
-  std::sort(ndx.begin(), ndx.end(), [position, &vposs](size_t n1, size_t n2) {
-    return abs(position - vposs[n1]) < abs(position - vposs[n2]);
-  });
-  if ((int)ndx.size() > CTH_MAXDST) ndx.resize(CTH_MAXDST);
-  std::sort(ndx.begin(), ndx.end(), [&vposs](size_t n1, size_t n2) { return vposs[n1] < vposs[n2]; });
-  std::vector<float> out;
-  out.reserve(2 * CTH_MAXDST);
-  if (ndx.size() && kDstFree < abs(position - vposs[ndx[0]])) out.push_back(vdsts[ndx[0]]);
-  int maxmainsep = 1;
-  float dstpen = 0.f;
-  for (size_t i = 1; i < ndx.size(); i++) {
-    int posep = vposs[ndx[i]] - vposs[ndx[i - 1]];
-    int mainsepp = abs(position - vposs[ndx[i - 1]]);
-    int mainsep = abs(position - vposs[ndx[i]]);
-    int mainsp2 = std::max(mainsepp, mainsep);
-    if (mainsep <= kDstFree) {
-      if (posep > 1 && maxmainsep < mainsp2) maxmainsep = mainsp2;
-      continue;
-    }
-    if ((posep > 1 && mainsp2 >= kDstNpos) || maxmainsep >= kDstNpos)
-      out.push_back(dstpen = kDstGapl);
-    else if (posep > 1 || maxmainsep > 1)
-      out.push_back(dstpen = kDstSep6);
-    for (int k = 2; k < posep && k < kDstSegm; k++) out.push_back(dstpen);
-    out.push_back(vdsts[ndx[i]]);
-    maxmainsep = 1;
-  }
-  if ((int)out.size() > CTH_MAXDST) {
-    if (position + CTH_MAXDST > L)
-      out.erase(out.begin(), out.begin() + (out.size() - CTH_MAXDST));
-    else
-      out.resize(CTH_MAXDST);
-  }
-  dgram[position] = out;
-}
-
 // drive the transform over a full list of pair lines (i<j, j-i>=3), as
 // adddist's ReadDistances does (adddist.cpp:269-367); dst(i,j) < 0 marks a
 // line whose distance is missing or below the probability threshold
@@ -177,7 +132,7 @@ void adddist_lists(int L, DstFn dst, std::vector<std::vector<float>>& dgram) {
       any = true;
       int r0 = i + 1;
       if (prevpos < r0) {
-        process_position(prevpos - 1, L, vdsts, vposs, tri, dgram);
+        cth_adddist_position(prevpos - 1, L, vdsts, vposs, tri, dgram);
         vdsts.clear(); vposs.clear();
         prevpos = r0;
       }
@@ -190,7 +145,7 @@ void adddist_lists(int L, DstFn dst, std::vector<std::vector<float>>& dgram) {
     }
   }
   (void)any;
-  if (L > 0) process_position(prevpos - 1, L, vdsts, vposs, tri, dgram);
+  if (L > 0) cth_adddist_position(prevpos - 1, L, vdsts, vposs, tri, dgram);
 }
 
 // approximately self-avoiding C-alpha chain with 3.8 A steps

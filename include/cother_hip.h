@@ -73,6 +73,19 @@ cth_pset* cth_read_binary_db(const char* path);
  * values are scaled integers (scale 10000 in the reference's files). */
 int cth_write_text_profiles(const cth_pset* ps, const char* path, int scale);
 
+/* Distance files (adddist: ReadDistances, GetPositionPairHelper,
+ * GetPairDistanceHelper and ProcessDistancesForPosition, src/adddist/
+ * adddist.cpp:267-651): lines "i j d.. [p..]" of residue pairs (1-based,
+ * ordered by i), the distance the mean of the given columns (1-based, >= 3;
+ * NULL / 0 columns: column 3), a probability after each distance when
+ * prbthrhld > 0.  cth_read_distances writes prolen lists of
+ * MAX_N_IR_DISTANCE_VALUES + 1 floats (count, then values);
+ * cth_add_distances stores them as profile k's distance lists, rounded as
+ * adddist's text output rounds them (PMProfileModel.cpp:2063-2067). */
+int cth_read_distances(const char* path, int prolen, const int* columns, int ncolumns, float prbthrhld,
+                       float* lists);
+int cth_add_distances(cth_pset* ps, int k, const char* path, const int* columns, int ncolumns, float prbthrhld);
+
 /* ---- context ------------------------------------------------------------------ */
 typedef struct cth_ctx cth_ctx;
 

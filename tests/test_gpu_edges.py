@@ -58,3 +58,17 @@ def test_long_profiles_with_self_hit(ctx, oracle):
     assert n >= 1
     hits, _, _ = ctx.search(eval=10.0)
     assert hits[0]["profile"] == 0 and hits[0]["alnlen"] > 1900
+
+
+def test_empty_database_and_profiles_without_distances(ctx, oracle):
+    q = cother_amd.synth(2, 0, lmin=30, lmax=80, dist_mode=1, sort_desc=0, seed=97)
+    empty = cother_amd.synth(0, 0, lmin=30, lmax=80, dist_mode=1, sort_desc=0, seed=98)
+    ctx.upload_queries(q)
+    ctx.upload_db(empty)
+    print("edges: empty database", flush=True)
+    hits, nh, st = ctx.search(eval=10.0)
+    assert len(hits) == 0 and list(nh) == [0, 0] and st.cells == 0
+    # templates without distance lists: the DDMS term is skipped (lists shorter than DSTSEGM)
+    nodist = cother_amd.synth(5, 0, lmin=30, lmax=200, dist_mode=2, sort_desc=0, seed=99)
+    print("edges: templates without distance lists", flush=True)
+    assert _compare(ctx, oracle, q, nodist, 100.0) >= 0
