@@ -58,6 +58,7 @@ extern "C" int cth_ctx_create(int device, const char* envdir, cth_ctx** out) {
   CTH_HIP_CHECK(hipSetDevice(device));
   cth_ctx* c = (cth_ctx*)calloc(1, sizeof(cth_ctx));
   if (!c) { cth_set_error("out of host memory"); return -1; }
+  new (&c->nshared) std::atomic<int>(0);
   c->device = device;
   int rc = cth_tables_load(envdir, &c->htab);
   if (rc) { cth_set_error("cannot load score tables from %s (code %d)", envdir, rc); free(c); return -1; }
@@ -83,8 +84,10 @@ extern "C" int cth_ctx_create_shared(cth_ctx* parent, cth_ctx** out) {
   CTH_HIP_CHECK(hipSetDevice(parent->device));
   cth_ctx* c = (cth_ctx*)calloc(1, sizeof(cth_ctx));
   if (!c) { cth_set_error("out of host memory"); return -1; }
+  new (&c->nshared) std::atomic<int>(0);
   c->device = parent->device;
   c->borrowed = 1;
+  c->parent = parent;
   c->htab = parent->htab;
   c->dtab = parent->dtab;
   c->q = parent->q;
@@ -94,6 +97,7 @@ extern "C" int cth_ctx_create_shared(cth_ctx* parent, cth_ctx** out) {
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
     cth_set_error("hipStreamCreate failed"); free(c); return -1;
   }
+  parent->nshared++;
   *out = c;
   return 0;
 }
@@ -114,6 +118,7 @@ extern "C" void cth_ctx_destroy(cth_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->st) (void)hipStreamSynchronize(c->st);
+  if (c->borrowed && c->parent) c->parent->nshared--;
   if (!c->borrowed) {
     cth_devset_free(&c->q);
     cth_devset_free(&c->db);
@@ -142,6 +147,7 @@ extern "C" int cth_ctx_shape(const cth_ctx* c, int* nqueries, const int32_t** ql
 
 extern "C" int cth_upload_queries(cth_ctx* c, const cth_pset* qs) {
   if (c->borrowed) { cth_set_error("upload on a shared context"); return -1; }
+  if (c->nshared > 0) { cth_set_error("upload while %d shared contexts read this one", c->nshared.load()); return -1; }
   CTH_HIP_CHECK(hipSetDevice(c->device));
   cth_devset_free(&c->q);
   c->has_q = 0;
@@ -153,6 +159,7 @@ extern "C" int cth_upload_queries(cth_ctx* c, const cth_pset* qs) {
 
 extern "C" int cth_upload_db(cth_ctx* c, const cth_pset* db) {
   if (c->borrowed) { cth_set_error("upload on a shared context"); return -1; }
+  if (c->nshared > 0) { cth_set_error("upload while %d shared contexts read this one", c->nshared.load()); return -1; }
   CTH_HIP_CHECK(hipSetDevice(c->device));
   cth_devset_free(&c->db);
   c->has_db = 0;

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include "cth_dev.h"
 #include "cth_devdb.h"
 #include "cth_tables.h"
@@ -27,6 +29,8 @@ struct cth_ctx {
   // a context made by cth_ctx_create_shared reads its parent's tables,
   // queries and database and frees only its own stream and scratch
   int borrowed;
+  cth_ctx* parent;             // the context a shared one reads from
+  std::atomic<int> nshared;    // live shared contexts reading this one's data
 };
 
 // release scratch buffers larger than keep bytes (the next allocation sizes them anew)

@@ -66,3 +66,16 @@ def test_request_over_budget_is_refused(ctx, sets):
             srv.submit(0)
     finally:
         srv.close()
+
+
+def test_uploads_refused_while_serving(ctx, sets):
+    q, db = sets
+    ctx.upload_queries(q)
+    ctx.upload_db(db)
+    srv = cother_amd.Server(ctx, nslots=2, mem_budget=1 << 34, chunk_cols=1 << 16)
+    try:
+        with pytest.raises(RuntimeError, match="shared"):
+            ctx.upload_db(db)
+    finally:
+        srv.close()
+    ctx.upload_db(db)  # allowed again once the server is gone
