@@ -118,7 +118,7 @@ def serve_phase(ctx, qs, db, args, seq_qps):
     n = args.serve_requests
     ql = qs.lens()
     order = [(args.warmup + s) % qs.nprof for s in range(n)]
-    rate = seq_qps * args.serve_load
+    rate = args.serve_rate if args.serve_rate > 0 else seq_qps * args.serve_load
     rng = np.random.default_rng(args.seed + 7)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n))
     srv = cother_amd.Server(ctx, nslots=args.serve_slots, mem_budget=64 << 30, chunk_cols=4 << 20, eval=args.eval)
@@ -134,7 +134,7 @@ def serve_phase(ctx, qs, db, args, seq_qps):
             tickets[k] = srv.submit(qi)
             ready[k].set()
 
-    log(f"serving: {n} requests, Poisson at {rate:.3f}/s ({args.serve_load}x the sequential rate), "
+    log(f"serving: {n} requests, Poisson at {rate:.3f}/s ({rate / seq_qps:.2f}x the sequential rate), "
         f"{args.serve_slots} slots")
     t_start = time.perf_counter()
     th = threading.Thread(target=submitter)
@@ -153,6 +153,7 @@ def serve_phase(ctx, qs, db, args, seq_qps):
     srv.close()
     lat = np.array(lat)
     return {"requests": n, "slots": args.serve_slots, "offered_qps": round(rate, 4),
+            "offered_load": round(rate / seq_qps, 3),
             "gcups": round(cells / makespan / 1e9, 4), "p50_ms": round(float(np.percentile(lat, 50)), 2),
             "p99_ms": round(float(np.percentile(lat, 99)), 2), "mean_queued_ms": round(float(np.mean(queued)), 2),
             "mem_budget_gib": 64, "chunk_cols": 4 << 20}
@@ -204,6 +205,8 @@ def main():
     ap.add_argument("--serve-load", type=float, default=1.5,
                     help="serving phase: offered load as a multiple of the sequential query rate "
                          "(above 1 the queue never drains, so the phase's GCUPS is the sustained rate)")
+    ap.add_argument("--serve-rate", type=float, default=0.0,
+                    help="serving phase: absolute arrival rate in queries/s (overrides --serve-load)")
     ap.add_argument("--serve-slots", type=int, default=2)
     ap.add_argument("--contraction", type=int, default=0, choices=(0, 1, 2),
                     help="log-odds term: 0 FP32 (the reference's arithmetic), 1 BF16 / 2 FP16 matrix-core variant")

@@ -35,3 +35,31 @@ def test_search_matches_oracle(ctx, oracle, budget):
             assert (got[f] == want[f]).all(), f
         for f in STATS:
             np.testing.assert_allclose(got[f], want[f], rtol=2e-6, atol=1e-30, err_msg=f)
+
+
+@pytest.mark.timeout(400)
+def test_search_at_the_cpu_runnable_size(ctx, oracle):
+    """One query of 150 positions (predicted distance map) against 2000
+    templates of 50-400 positions (C-alpha chains), the workload the CPU path
+    can run whole (~7e7 cells), plus the query itself as template 0 so a long
+    alignment is among the hits: the hit list equals the oracle's."""
+    q = cother_amd.synth(1, 2, lmin=150, lmax=150, dist_mode=0, sort_desc=0, seed=57)
+    rest = cother_amd.synth(2000, 0, lmin=50, lmax=400, dist_mode=1, sort_desc=1, seed=58)
+    db = cother_amd.concat([cother_amd.subset(q, 0, 1), rest])
+    ctx.upload_queries(q)
+    ctx.upload_db(db)
+    print(f"search: Lq=150 vs {db.nprof} templates ({db.npos} positions)", flush=True)
+    hits, nh, st = ctx.search(eval=10.0)
+    want = oracle.search_query(q, 0, db, eval_=10.0)
+    print(f"{len(hits)} hits (oracle {len(want)})", flush=True)
+    assert len(hits) == len(want) >= 1
+    assert hits[0]["profile"] == 0 and hits[0]["alnlen"] >= 140
+    for f in FIELDS:
+        assert (hits[f] == want[f]).all(), f
+    for f in STATS:
+        np.testing.assert_allclose(hits[f], want[f], rtol=2e-6, atol=1e-30, err_msg=f)
+    # the score matrix over the first 60 templates, bit for bit
+    sc, md = ctx.score_block(0, 0, 60)
+    osc, omd = oracle.score_matrix(q, 0, db, 0, 60)
+    assert (sc.view(np.int32) == osc.view(np.int32)).all()
+    assert (md.view(np.int32) == omd.view(np.int32)).all()
