@@ -63,10 +63,14 @@ __device__ __forceinline__ bool coords_intersect(uint32_t c1, uint32_t c2) {
 //  m2 <= s < m1, s intersecting neither m0 nor m1: s becomes m2.
 // An entry whose value is 0 may keep stale coordinates: every intersection
 // test with it only decides between two zeros, and coordinates of list
-// indices (< 99) never intersect the initial 0xffffffff.
+// indices (< 99) never intersect the initial 0xffffffff.  i0, i1: whether s
+// intersects the current m0, m1.  The register changes only if
+// register_changes(); otherwise every select below keeps the old value.
+__device__ __forceinline__ bool register_changes(float m0, float m1, float s, bool i0, bool i1) {
+  return (m0 <= s) | (!i0 & ((m1 <= s) | !i1));
+}
 __device__ __forceinline__ void register_max(float& m0, float& m1, float& m2, uint32_t& c0, uint32_t& c1,
-                                             uint32_t& c2, float s, uint32_t sc) {
-  const bool i0 = coords_intersect(sc, c0), i1 = coords_intersect(sc, c1);
+                                             uint32_t& c2, float s, uint32_t sc, bool i0, bool i1) {
   const bool a = m0 <= s, b1 = m1 <= s;
   const bool b = !a & b1 & !i0;
   const bool c = !a & !b1 & !i0 & !i1;
@@ -128,7 +132,12 @@ __device__ __forceinline__ uint32_t mkc(int qb, int qe, int pb, int pe) {
 // one 4-cell step, cells K0+1..K0+4 relative to the base ib, with pair
 // scores S0 (cell K0, the previous step's last cell), A2, A4 (cells K0+2 and
 // K0+4); the begin and end of the step's best are recovered from the take
-// flags only when it is registered
+// flags only when it can be registered.  Two levels: the step's best is a
+// candidate when it reaches max(m2, FLT_MIN); a candidate changes the
+// register only when it passes the intersection tests (most candidates
+// intersect m0 or m1 while scoring below them: segments of neighbouring
+// diagonals through the same region), so the selects run for a wavefront
+// only when one of its lanes really changes its register.
 #define CTH_DDMS_4CELL(K0, S0, A2, A4, QOFF, POFF)                            \
   {                                                                           \
     float scom;                                                               \
@@ -144,8 +153,12 @@ __device__ __forceinline__ uint32_t mkc(int qb, int qe, int pb, int pe) {
       bb = take3 ? bb3 : bb; be = take3 ? (K0) + 3 : be;                      \
       bb = take4 ? bb4 : bb; be = take4 ? (K0) + 4 : be;                      \
       bb += ib; be += ib;                                                     \
-      register_max(m0, m1, m2, c0, c1, c2, scom, mkc(bb + (QOFF), be + (QOFF), bb + (POFF), be + (POFF))); \
-      m2f = fmaxf(m2, FLT_MIN);                                               \
+      const uint32_t sc = mkc(bb + (QOFF), be + (QOFF), bb + (POFF), be + (POFF)); \
+      const bool i0 = coords_intersect(sc, c0), i1 = coords_intersect(sc, c1); \
+      if (register_changes(m0, m1, scom, i0, i1)) {                           \
+        register_max(m0, m1, m2, c0, c1, c2, scom, sc, i0, i1);               \
+        m2f = fmaxf(m2, FLT_MIN);                                             \
+      }                                                                       \
     }                                                                         \
   }
 
@@ -157,7 +170,7 @@ __device__ __forceinline__ float pair(const float* __restrict__ dmsT, int qb, in
 
 // q: the query list as table byte offsets (entry k = 88 * value k, entry 0 =
 // the list length); t: the template list (byte 0 = length, then values)
-__device__ float ddms_cell(const float* __restrict__ dmsT, const int32_t* __restrict__ q,
+__device__ __forceinline__ float ddms_cell(const float* __restrict__ dmsT, const int32_t* __restrict__ q,
                            const uint8_t* __restrict__ t) {
   const int len1 = q[0], len2 = t[0];
   if (len1 < CTH_DSTSEGM || len2 < CTH_DSTSEGM) return 0.0f;
