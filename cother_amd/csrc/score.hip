@@ -43,13 +43,15 @@ __device__ __forceinline__ float ss_lookup(const float* ss, int row, int col) {
   return ss[col + 2];
 }
 
-__device__ __forceinline__ bool coords_intersect(uint32_t c1, uint32_t c2) {
-  // byte-wise max/min of [QB QE PB PE] (CuBatchScoreMatrix_ddms2s.cuh:141-153)
-  const uint32_t qb = max(c1 >> 24, c2 >> 24);
-  const uint32_t qe = min((c1 >> 16) & 0xff, (c2 >> 16) & 0xff);
-  const uint32_t pb = max((c1 >> 8) & 0xff, (c2 >> 8) & 0xff);
-  const uint32_t pe = min(c1 & 0xff, c2 & 0xff);
-  return qe >= qb || pe >= pb;
+// whether the segment [qb, qe] x [pb, pe] intersects the one whose packed
+// coordinates [QB QE PB PE] are c: byte-wise max of the begins <= min of the
+// ends on either axis (CuBatchScoreMatrix_ddms2s.cuh:141-153)
+__device__ __forceinline__ bool coords_intersect(uint32_t qb, uint32_t qe, uint32_t pb, uint32_t pe, uint32_t c) {
+  const uint32_t b0 = max(qb, c >> 24);
+  const uint32_t e0 = min(qe, (c >> 16) & 0xff);
+  const uint32_t b1 = max(pb, (c >> 8) & 0xff);
+  const uint32_t e1 = min(pe, c & 0xff);
+  return e0 >= b0 || e1 >= b1;
 }
 
 // RegisterMaxDDMScore (CuBatchScoreMatrix_ddms2s.cuh:168-261): keep the
@@ -87,9 +89,9 @@ __device__ __forceinline__ void register_max(float& m0, float& m1, float& m2, ui
   c2 = d2;
 }
 
+// [QB QE PB PE] packed; each is a list index (< CTH_MAXDST), so no masking
 __device__ __forceinline__ uint32_t mkc(int qb, int qe, int pb, int pe) {
-  return ((uint32_t)(qb & 0xff) << 24) | ((uint32_t)(qe & 0xff) << 16) | ((uint32_t)(pb & 0xff) << 8) |
-         (uint32_t)(pe & 0xff);
+  return ((uint32_t)qb << 24) | ((uint32_t)qe << 16) | ((uint32_t)pb << 8) | (uint32_t)pe;
 }
 
 // CalcDDMS2ScoreSMEMh2<2,true,2> (cusco/CuBatchScoreMatrix_ddms2s_h2.cuh:435-500):
@@ -112,7 +114,8 @@ __device__ __forceinline__ uint32_t mkc(int qb, int qe, int pb, int pe) {
 // reference's rounding of segsco + 0.5f * (a + b).  The top-3 register is
 // entered only when the step's best can displace its smallest entry:
 // scom >= max(m2, FLT_MIN) (positive sums of the pair scores are far above
-// FLT_MIN, so this is scom > 0 && scom >= m2).
+// FLT_MIN, so this is scom > 0 && scom >= m2; m2 is finite, so the max is
+// the median of m2, FLT_MIN and FLT_MAX, one instruction).
 #define CTH_DDMS_FIRST(K, SUM)                                                \
   {                                                                           \
     segr = (segsco != 0.0f) ? segr : (K);                                     \
@@ -153,11 +156,12 @@ __device__ __forceinline__ uint32_t mkc(int qb, int qe, int pb, int pe) {
       bb = take3 ? bb3 : bb; be = take3 ? (K0) + 3 : be;                      \
       bb = take4 ? bb4 : bb; be = take4 ? (K0) + 4 : be;                      \
       bb += ib; be += ib;                                                     \
-      const uint32_t sc = mkc(bb + (QOFF), be + (QOFF), bb + (POFF), be + (POFF)); \
-      const bool i0 = coords_intersect(sc, c0), i1 = coords_intersect(sc, c1); \
+      const uint32_t qb = bb + (QOFF), qe = be + (QOFF), pb = bb + (POFF), pe = be + (POFF); \
+      const bool i0 = coords_intersect(qb, qe, pb, pe, c0);                   \
+      const bool i1 = coords_intersect(qb, qe, pb, pe, c1);                   \
       if (register_changes(m0, m1, scom, i0, i1)) {                           \
-        register_max(m0, m1, m2, c0, c1, c2, scom, sc, i0, i1);               \
-        m2f = fmaxf(m2, FLT_MIN);                                             \
+        register_max(m0, m1, m2, c0, c1, c2, scom, mkc(qb, qe, pb, pe), i0, i1); \
+        m2f = __builtin_amdgcn_fmed3f(m2, FLT_MIN, FLT_MAX);                  \
       }                                                                       \
     }                                                                         \
   }
