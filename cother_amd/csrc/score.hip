@@ -100,62 +100,63 @@ __device__ __forceinline__ uint32_t mkc(int qb, int qe, int pb, int pe) {
 // are exact pair scores and cells 1 and 3 their means; per 4-cell step the
 // best qualifying segment (length >= DSTSEGM) goes to the top-3 register.
 //
-// Branch-free form of the reference's per-cell update.  A segment's begin
-// index is kept on the scanned axis only (the other is that index plus the
-// diagonal offset) and relative to the base of the current pair of 4-cell
-// steps, so the cell indices and length bounds are the literals 1..8.
-// "Score > 0 and better than the step's best" is "better than the step's
-// best" for cells 2-4 of a step because the best starts at 0, and for its
-// cell 1 the length test alone decides: a zero taken there is never
-// registered.  The clamp at 0 is a max (it differs from the reference's
-// select only in the sign of a zero sum, and a zero is never registered).
-// The mean of two pair scores enters as fmaf(0.5, a + b, segsco): 0.5 (a + b)
-// is exact, so the one rounding of the fused operation equals the
-// reference's rounding of segsco + 0.5f * (a + b).  The top-3 register is
-// entered only when the step's best can displace its smallest entry:
-// scom >= max(m2, FLT_MIN) (positive sums of the pair scores are far above
-// FLT_MIN, so this is scom > 0 && scom >= m2; m2 is finite, so the max is
-// the median of m2, FLT_MIN and FLT_MAX, one instruction).
-#define CTH_DDMS_FIRST(K, SUM)                                                \
+// The per-cell update in arithmetic that issues on the VALU's second
+// (add/mul) slot: compares, selects and min/max issue one per cycle, plain
+// FP32 adds and multiplies (source modifiers and clamp included) two.  Per
+// cell, with dd = the segment's begin minus the cell index (an integer
+// value, <= 0) and big = 2^126:
+//   nz     = clamp(segsco * big)        1 when the previous sum is > 0, else 0
+//                                       (positive sums of pair scores are far
+//                                       above 2^-126)
+//   dd     = fma(nz, dd, -nz)           dd - 1, or 0 where the segment restarts
+//   t      = segsco + pair score        (the mean of two pair scores enters as
+//                                       fmaf(0.5, a + b, segsco): 0.5 (a + b)
+//                                       is exact, so the one rounding equals
+//                                       the reference's of segsco + 0.5f*(a+b))
+//   segsco = (t + |t|) * 0.5            max(t, 0), exact (t + |t| is 2t or 0)
+//   cand   = segsco * clamp(-dd - 4)    segsco if the segment has at least
+//                                       DSTSEGM = 6 cells (dd <= -5), else 0
+// and the step's best is the max of its four candidates: the reference takes
+// a cell when its sum exceeds the best so far, i.e. the first cell reaching
+// the max.  The clamp at 0 differs from the reference's select only in the
+// sign of a zero sum, and a zero is never registered.  The top-3 register
+// is entered only when the step's best can displace its smallest entry:
+// scom >= max(m2, FLT_MIN) (positive sums are far above FLT_MIN, so this is
+// scom > 0 && scom >= m2; m2 is finite, so the max is the median of m2,
+// FLT_MIN and FLT_MAX, one instruction).
+#define CTH_DDMS_CELL(SUM, DOUT, COUT)                                        \
   {                                                                           \
-    segr = (segsco != 0.0f) ? segr : (K);                                     \
-    segsco = fmaxf(SUM, 0.0f);                                                \
-    take1 = segr <= (K) + 1 - CTH_DSTSEGM;                                    \
-    scom = take1 ? segsco : 0.0f;                                             \
-    bb1 = segr;                                                               \
-  }
-#define CTH_DDMS_NEXT(K, SUM, TAKE, BB)                                       \
-  {                                                                           \
-    segr = (segsco != 0.0f) ? segr : (K);                                     \
-    segsco = fmaxf(SUM, 0.0f);                                                \
-    TAKE = (scom < segsco) & (segr <= (K) + 1 - CTH_DSTSEGM);                 \
-    scom = TAKE ? segsco : scom;                                              \
-    BB = segr;                                                                \
+    const float nz = __builtin_amdgcn_fmed3f(segsco * big, 0.0f, 1.0f);       \
+    dd = fmaf(nz, dd, -nz);                                                   \
+    const float t = SUM;                                                      \
+    segsco = (t + fabsf(t)) * 0.5f;                                           \
+    COUT = segsco * __builtin_amdgcn_fmed3f(-dd - 4.0f, 0.0f, 1.0f);          \
+    DOUT = dd;                                                                \
   }
 // one 4-cell step, cells K0+1..K0+4 relative to the base ib, with pair
 // scores S0 (cell K0, the previous step's last cell), A2, A4 (cells K0+2 and
-// K0+4); the begin and end of the step's best are recovered from the take
-// flags only when it can be registered.  Two levels: the step's best is a
-// candidate when it reaches max(m2, FLT_MIN); a candidate changes the
+// K0+4); the begin and end of the step's best are recovered from the cells'
+// candidates only when it can be registered.  Two levels: the step's best is
+// a candidate when it reaches max(m2, FLT_MIN); a candidate changes the
 // register only when it passes the intersection tests (most candidates
 // intersect m0 or m1 while scoring below them: segments of neighbouring
 // diagonals through the same region), so the selects run for a wavefront
 // only when one of its lanes really changes its register.
 #define CTH_DDMS_4CELL(K0, S0, A2, A4, QOFF, POFF)                            \
   {                                                                           \
-    float scom;                                                               \
-    bool take1, take2, take3, take4;                                          \
-    int bb1, bb2, bb3, bb4;                                                   \
-    CTH_DDMS_FIRST((K0) + 1, fmaf(0.5f, S0 + A2, segsco))                     \
-    CTH_DDMS_NEXT((K0) + 2, segsco + A2, take2, bb2)                          \
-    CTH_DDMS_NEXT((K0) + 3, fmaf(0.5f, A2 + A4, segsco), take3, bb3)          \
-    CTH_DDMS_NEXT((K0) + 4, segsco + A4, take4, bb4)                          \
+    float cd1, cd2, cd3, cd4, dd1, dd2, dd3, dd4;                             \
+    CTH_DDMS_CELL(fmaf(0.5f, S0 + A2, segsco), dd1, cd1)                      \
+    CTH_DDMS_CELL(segsco + A2, dd2, cd2)                                      \
+    CTH_DDMS_CELL(fmaf(0.5f, A2 + A4, segsco), dd3, cd3)                      \
+    CTH_DDMS_CELL(segsco + A4, dd4, cd4)                                      \
+    const float scom = fmaxf(fmaxf(cd1, cd2), fmaxf(cd3, cd4));               \
     if (scom >= m2f) {                                                        \
-      int bb = take1 ? bb1 : 0, be = (K0) + 1;                                \
-      bb = take2 ? bb2 : bb; be = take2 ? (K0) + 2 : be;                      \
-      bb = take3 ? bb3 : bb; be = take3 ? (K0) + 3 : be;                      \
-      bb = take4 ? bb4 : bb; be = take4 ? (K0) + 4 : be;                      \
-      bb += ib; be += ib;                                                     \
+      int k = 4;                                                              \
+      float dk = dd4;                                                         \
+      k = cd3 == scom ? 3 : k; dk = cd3 == scom ? dd3 : dk;                   \
+      k = cd2 == scom ? 2 : k; dk = cd2 == scom ? dd2 : dk;                   \
+      k = cd1 == scom ? 1 : k; dk = cd1 == scom ? dd1 : dk;                   \
+      const int be = ib + (K0) + k, bb = be + (int)dk;                        \
       const uint32_t qb = bb + (QOFF), qe = be + (QOFF), pb = bb + (POFF), pe = be + (POFF); \
       const bool i0 = coords_intersect(qb, qe, pb, pe, c0);                   \
       const bool i1 = coords_intersect(qb, qe, pb, pe, c1);                   \
@@ -180,13 +181,17 @@ __device__ __forceinline__ float ddms_cell(const float* __restrict__ dmsT, const
   if (len1 < CTH_DSTSEGM || len2 < CTH_DSTSEGM) return 0.0f;
   float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f, m2f = FLT_MIN;
   uint32_t c0 = 0xffffffffu, c1 = 0xffffffffu, c2 = 0xffffffffu;
+  // 2^126 in a vector register: as a scalar operand it would keep the
+  // multiply off the second issue slot
+  float big;
+  asm("v_mov_b32 %0, 0x7e800000" : "=v"(big));
   // upper part: diagonal j pairs query index i with template index j + i
   for (int j = len2 >> 2; 0 <= j; j -= 2) {
     const uint8_t* tj = t + 1 + j;
     const int iend = min(len1, len2 - j);
     float s0 = pair(dmsT, q[1], tj[0]);
     float segsco = fmaxf(s0, 0.0f);
-    int segr = 0;
+    float dd = 0.0f;
     float s2 = pair(dmsT, q[1 + 2], tj[2]);
     float s4 = pair(dmsT, q[1 + 4], tj[4]);
     for (int ib = 0; ib < iend; ib += 8) {
@@ -198,7 +203,6 @@ __device__ __forceinline__ float ddms_cell(const float* __restrict__ dmsT, const
       const float p2 = pair(dmsT, q[1 + ib + 10], tj[ib + 10]);
       const float p4 = pair(dmsT, q[1 + ib + 12], tj[ib + 12]);
       if (ib + 4 < iend) CTH_DDMS_4CELL(4, s4, n2, n4, 0, j)
-      segr -= 8;
       s0 = n4;
       s2 = p2;
       s4 = p4;
@@ -210,7 +214,7 @@ __device__ __forceinline__ float ddms_cell(const float* __restrict__ dmsT, const
     const int jend = min(len2, len1 - i);
     float s0 = pair(dmsT, qi[0], t[1]);
     float segsco = fmaxf(s0, 0.0f);
-    int segr = 0;
+    float dd = 0.0f;
     float s2 = pair(dmsT, qi[2], t[1 + 2]);
     float s4 = pair(dmsT, qi[4], t[1 + 4]);
     for (int ib = 0; ib < jend; ib += 8) {
@@ -220,7 +224,6 @@ __device__ __forceinline__ float ddms_cell(const float* __restrict__ dmsT, const
       const float p2 = pair(dmsT, qi[ib + 10], t[1 + ib + 10]);
       const float p4 = pair(dmsT, qi[ib + 12], t[1 + ib + 12]);
       if (ib + 4 < jend) CTH_DDMS_4CELL(4, s4, n2, n4, i, 0)
-      segr -= 8;
       s0 = n4;
       s2 = p2;
       s4 = p4;
@@ -233,8 +236,7 @@ __device__ __forceinline__ float ddms_cell(const float* __restrict__ dmsT, const
   s = fmaf(CTH_DDMS_SLOPE * CTH_DDMS_SCALE, s, CTH_DDMS_ICEPT);
   return s * CTH_DDMSWGT;
 }
-#undef CTH_DDMS_FIRST
-#undef CTH_DDMS_NEXT
+#undef CTH_DDMS_CELL
 #undef CTH_DDMS_4CELL
 
 // kPreInit: the log-odds term is already in scores (the reduced-precision
