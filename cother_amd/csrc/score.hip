@@ -102,35 +102,38 @@ __device__ __forceinline__ uint32_t mkc(int qb, int qe, int pb, int pe) {
 //
 // The per-cell update in arithmetic that issues on the VALU's second
 // (add/mul) slot: compares, selects and min/max issue one per cycle, plain
-// FP32 adds and multiplies (source modifiers and clamp included) two.  Per
-// cell, with dd = the segment's begin minus the cell index (an integer
-// value, <= 0) and big = 2^126:
-//   nz     = clamp(segsco * big)        1 when the previous sum is > 0, else 0
-//                                       (positive sums of pair scores are far
-//                                       above 2^-126)
-//   dd     = fma(nz, dd, -nz)           dd - 1, or 0 where the segment restarts
-//   t      = segsco + pair score        (the mean of two pair scores enters as
-//                                       fmaf(0.5, a + b, segsco): 0.5 (a + b)
-//                                       is exact, so the one rounding equals
-//                                       the reference's of segsco + 0.5f*(a+b))
-//   segsco = (t + |t|) * 0.5            max(t, 0), exact (t + |t| is 2t or 0)
-//   cand   = segsco * clamp(-dd - 4)    segsco if the segment has at least
-//                                       DSTSEGM = 6 cells (dd <= -5), else 0
+// FP32 adds, multiplies and fmas (source modifiers and clamp included) two.
+// The running sum is kept doubled, u = 2 segsco (exact), so that the next
+// sum is one fma.  Per cell, with x the cell's pair score (or, for the
+// interpolated cells, 0.5 (a + b): the reference's 0.5f * (a + b), exact),
+// dd = the segment's begin minus the cell index (an integer value, <= 0) and
+// big = 2^126:
+//   nz   = clamp(u * big)          1 when the previous sum is > 0, else 0
+//                                  (positive sums of pair scores are far
+//                                  above 2^-126)
+//   dd   = fma(nz, dd, -nz)        dd - 1, or 0 where the segment restarts
+//   t    = fma(0.5, u, x)          segsco + x with the reference's one
+//                                  rounding (0.5 u is exact)
+//   u    = t + |t|                 2 max(t, 0), exact
+//   cand = u * clamp(-dd - 4)      u if the segment has at least DSTSEGM = 6
+//                                  cells (dd <= -5), else 0
 // and the step's best is the max of its four candidates: the reference takes
 // a cell when its sum exceeds the best so far, i.e. the first cell reaching
 // the max.  The clamp at 0 differs from the reference's select only in the
 // sign of a zero sum, and a zero is never registered.  The top-3 register
-// is entered only when the step's best can displace its smallest entry:
+// holds doubled sums too (doubling commutes with every comparison and with
+// the rounding of the final sum, which is halved exactly).  It is entered
+// only when the step's best can displace its smallest entry:
 // scom >= max(m2, FLT_MIN) (positive sums are far above FLT_MIN, so this is
 // scom > 0 && scom >= m2; m2 is finite, so the max is the median of m2,
 // FLT_MIN and FLT_MAX, one instruction).
-#define CTH_DDMS_CELL(SUM, DOUT, COUT)                                        \
+#define CTH_DDMS_CELL(X, DOUT, COUT)                                          \
   {                                                                           \
-    const float nz = __builtin_amdgcn_fmed3f(segsco * big, 0.0f, 1.0f);       \
+    const float nz = __builtin_amdgcn_fmed3f(u * big, 0.0f, 1.0f);            \
     dd = fmaf(nz, dd, -nz);                                                   \
-    const float t = SUM;                                                      \
-    segsco = (t + fabsf(t)) * 0.5f;                                           \
-    COUT = segsco * __builtin_amdgcn_fmed3f(-dd - 4.0f, 0.0f, 1.0f);          \
+    const float t = fmaf(0.5f, u, X);                                         \
+    u = t + fabsf(t);                                                         \
+    COUT = u * __builtin_amdgcn_fmed3f(-dd - 4.0f, 0.0f, 1.0f);               \
     DOUT = dd;                                                                \
   }
 // one 4-cell step, cells K0+1..K0+4 relative to the base ib, with pair
@@ -145,10 +148,10 @@ __device__ __forceinline__ uint32_t mkc(int qb, int qe, int pb, int pe) {
 #define CTH_DDMS_4CELL(K0, S0, A2, A4, QOFF, POFF)                            \
   {                                                                           \
     float cd1, cd2, cd3, cd4, dd1, dd2, dd3, dd4;                             \
-    CTH_DDMS_CELL(fmaf(0.5f, S0 + A2, segsco), dd1, cd1)                      \
-    CTH_DDMS_CELL(segsco + A2, dd2, cd2)                                      \
-    CTH_DDMS_CELL(fmaf(0.5f, A2 + A4, segsco), dd3, cd3)                      \
-    CTH_DDMS_CELL(segsco + A4, dd4, cd4)                                      \
+    CTH_DDMS_CELL((S0 + A2) * 0.5f, dd1, cd1)                                 \
+    CTH_DDMS_CELL(A2, dd2, cd2)                                               \
+    CTH_DDMS_CELL((A2 + A4) * 0.5f, dd3, cd3)                                 \
+    CTH_DDMS_CELL(A4, dd4, cd4)                                               \
     const float scom = fmaxf(fmaxf(cd1, cd2), fmaxf(cd3, cd4));               \
     if (scom >= m2f) {                                                        \
       int k = 4;                                                              \
@@ -190,7 +193,7 @@ __device__ __forceinline__ float ddms_cell(const float* __restrict__ dmsT, const
     const uint8_t* tj = t + 1 + j;
     const int iend = min(len1, len2 - j);
     float s0 = pair(dmsT, q[1], tj[0]);
-    float segsco = fmaxf(s0, 0.0f);
+    float u = s0 + fabsf(s0);  // 2 max(s0, 0)
     float dd = 0.0f;
     float s2 = pair(dmsT, q[1 + 2], tj[2]);
     float s4 = pair(dmsT, q[1 + 4], tj[4]);
@@ -213,7 +216,7 @@ __device__ __forceinline__ float ddms_cell(const float* __restrict__ dmsT, const
     const int32_t* qi = q + 1 + i;
     const int jend = min(len2, len1 - i);
     float s0 = pair(dmsT, qi[0], t[1]);
-    float segsco = fmaxf(s0, 0.0f);
+    float u = s0 + fabsf(s0);  // 2 max(s0, 0)
     float dd = 0.0f;
     float s2 = pair(dmsT, qi[2], t[1 + 2]);
     float s4 = pair(dmsT, qi[4], t[1 + 4]);
@@ -233,6 +236,7 @@ __device__ __forceinline__ float ddms_cell(const float* __restrict__ dmsT, const
   s += m0;
   s += m1;
   s += m2;
+  s *= 0.5f;  // the register holds doubled sums
   s = fmaf(CTH_DDMS_SLOPE * CTH_DDMS_SCALE, s, CTH_DDMS_ICEPT);
   return s * CTH_DDMSWGT;
 }
@@ -269,6 +273,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
   }
   __syncthreads();
   if (row >= lq || col >= ncols) return;
+
+  // DDMS first, while nothing else is live: the wavefront's query list (one
+  // row per wavefront) by scalar loads
+  const int urow = __builtin_amdgcn_readfirstlane(row);
+  const float d = ddms_cell(L.dms, qoff + (size_t)(q0 + urow) * CTH_DDSTRIDE, &L.tl[lane * kListPitch]);
 
   const int64_t qi = q0 + row;
   const int64_t tj = col0 + col;
@@ -333,9 +342,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
   s = s + sc;
   float ms = fmaf(-CTH_CVSSHIFT, CTH_CVSWGT, sc);
 
-  // the wavefront's query list (one row per wavefront) by scalar loads
-  const int urow = __builtin_amdgcn_readfirstlane(row);
-  const float d = ddms_cell(L.dms, qoff + (size_t)(q0 + urow) * CTH_DDSTRIDE, &L.tl[lane * kListPitch]);
   if (d != 0.0f) {
     s = s + d;
     ms = ms + fmaf(-CTH_DDMSSHIFT, CTH_DDMSWGT, d);
