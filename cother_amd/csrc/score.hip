@@ -33,6 +33,7 @@ constexpr int kListPitch = 132; // LDS bytes per template list
 struct ScoreLds {
   uint8_t tl[kCols * kListPitch];
   float dms[22 * 22];
+  float dmsh[22 * 22];  // the pair scores halved (exact), right after dms
   float ss[467];
   float cvvals[CTH_CVMAXCARD];
 };
@@ -137,20 +138,22 @@ __device__ __forceinline__ uint32_t mkc(int qb, int qe, int pb, int pe) {
     DOUT = dd;                                                                \
   }
 // one 4-cell step, cells K0+1..K0+4 relative to the base ib, with pair
-// scores S0 (cell K0, the previous step's last cell), A2, A4 (cells K0+2 and
-// K0+4); the begin and end of the step's best are recovered from the cells'
+// scores A2, A4 (cells K0+2 and K0+4) and the halves H0, H2, H4 of those of
+// cells K0 (the previous step's last cell), K0+2 and K0+4: the reference's
+// interpolated sum 0.5f * (a + b) is h(a) + h(b), halving being exact and
+// commuting with the rounding of the sum; the begin and end of the step's best are recovered from the cells'
 // candidates only when it can be registered.  Two levels: the step's best is
 // a candidate when it reaches max(m2, FLT_MIN); a candidate changes the
 // register only when it passes the intersection tests (most candidates
 // intersect m0 or m1 while scoring below them: segments of neighbouring
 // diagonals through the same region), so the selects run for a wavefront
 // only when one of its lanes really changes its register.
-#define CTH_DDMS_4CELL(K0, S0, A2, A4, QOFF, POFF)                            \
+#define CTH_DDMS_4CELL(K0, H0, A2, H2, A4, H4, QOFF, POFF)                    \
   {                                                                           \
     float cd1, cd2, cd3, cd4, dd1, dd2, dd3, dd4;                             \
-    CTH_DDMS_CELL((S0 + A2) * 0.5f, dd1, cd1)                                 \
+    CTH_DDMS_CELL(H0 + H2, dd1, cd1)                                          \
     CTH_DDMS_CELL(A2, dd2, cd2)                                               \
-    CTH_DDMS_CELL((A2 + A4) * 0.5f, dd3, cd3)                                 \
+    CTH_DDMS_CELL(H2 + H4, dd3, cd3)                                          \
     CTH_DDMS_CELL(A4, dd4, cd4)                                               \
     const float scom = fmaxf(fmaxf(cd1, cd2), fmaxf(cd3, cd4));               \
     if (scom >= m2f) {                                                        \
@@ -175,6 +178,11 @@ __device__ __forceinline__ uint32_t mkc(int qb, int qe, int pb, int pe) {
 __device__ __forceinline__ float pair(const float* __restrict__ dmsT, int qb, int tv) {
   return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(dmsT) + qb + (tv << 2));
 }
+// the same pair score halved, from the table that follows dmsT (same address
+// plus a constant offset)
+__device__ __forceinline__ float pairh(const float* __restrict__ dmsT, int qb, int tv) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(dmsT + 22 * 22) + qb + (tv << 2));
+}
 
 // q: the query list as table byte offsets (entry k = 88 * value k, entry 0 =
 // the list length); t: the template list (byte 0 = length, then values)
@@ -192,44 +200,50 @@ __device__ __forceinline__ float ddms_cell(const float* __restrict__ dmsT, const
   for (int j = len2 >> 2; 0 <= j; j -= 2) {
     const uint8_t* tj = t + 1 + j;
     const int iend = min(len1, len2 - j);
-    float s0 = pair(dmsT, q[1], tj[0]);
+    const float s0 = pair(dmsT, q[1], tj[0]);
     float u = s0 + fabsf(s0);  // 2 max(s0, 0)
     float dd = 0.0f;
-    float s2 = pair(dmsT, q[1 + 2], tj[2]);
-    float s4 = pair(dmsT, q[1 + 4], tj[4]);
+    float h0 = pairh(dmsT, q[1], tj[0]);
+    float s2 = pair(dmsT, q[1 + 2], tj[2]), h2 = pairh(dmsT, q[1 + 2], tj[2]);
+    float s4 = pair(dmsT, q[1 + 4], tj[4]), h4 = pairh(dmsT, q[1 + 4], tj[4]);
     for (int ib = 0; ib < iend; ib += 8) {
       // two steps per iteration; the pair scores of the step after are loaded
       // one step ahead (lists are padded)
-      const float n2 = pair(dmsT, q[1 + ib + 6], tj[ib + 6]);
-      const float n4 = pair(dmsT, q[1 + ib + 8], tj[ib + 8]);
-      CTH_DDMS_4CELL(0, s0, s2, s4, 0, j)
-      const float p2 = pair(dmsT, q[1 + ib + 10], tj[ib + 10]);
-      const float p4 = pair(dmsT, q[1 + ib + 12], tj[ib + 12]);
-      if (ib + 4 < iend) CTH_DDMS_4CELL(4, s4, n2, n4, 0, j)
-      s0 = n4;
+      const float n2 = pair(dmsT, q[1 + ib + 6], tj[ib + 6]), nh2 = pairh(dmsT, q[1 + ib + 6], tj[ib + 6]);
+      const float n4 = pair(dmsT, q[1 + ib + 8], tj[ib + 8]), nh4 = pairh(dmsT, q[1 + ib + 8], tj[ib + 8]);
+      CTH_DDMS_4CELL(0, h0, s2, h2, s4, h4, 0, j)
+      const float p2 = pair(dmsT, q[1 + ib + 10], tj[ib + 10]), ph2 = pairh(dmsT, q[1 + ib + 10], tj[ib + 10]);
+      const float p4 = pair(dmsT, q[1 + ib + 12], tj[ib + 12]), ph4 = pairh(dmsT, q[1 + ib + 12], tj[ib + 12]);
+      if (ib + 4 < iend) CTH_DDMS_4CELL(4, h4, n2, nh2, n4, nh4, 0, j)
+      h0 = nh4;
       s2 = p2;
+      h2 = ph2;
       s4 = p4;
+      h4 = ph4;
     }
   }
   // lower part: diagonal i pairs query index i + j with template index j
   for (int i = len1 >> 2; 0 < i; i -= 2) {
     const int32_t* qi = q + 1 + i;
     const int jend = min(len2, len1 - i);
-    float s0 = pair(dmsT, qi[0], t[1]);
+    const float s0 = pair(dmsT, qi[0], t[1]);
     float u = s0 + fabsf(s0);  // 2 max(s0, 0)
     float dd = 0.0f;
-    float s2 = pair(dmsT, qi[2], t[1 + 2]);
-    float s4 = pair(dmsT, qi[4], t[1 + 4]);
+    float h0 = pairh(dmsT, qi[0], t[1]);
+    float s2 = pair(dmsT, qi[2], t[1 + 2]), h2 = pairh(dmsT, qi[2], t[1 + 2]);
+    float s4 = pair(dmsT, qi[4], t[1 + 4]), h4 = pairh(dmsT, qi[4], t[1 + 4]);
     for (int ib = 0; ib < jend; ib += 8) {
-      const float n2 = pair(dmsT, qi[ib + 6], t[1 + ib + 6]);
-      const float n4 = pair(dmsT, qi[ib + 8], t[1 + ib + 8]);
-      CTH_DDMS_4CELL(0, s0, s2, s4, i, 0)
-      const float p2 = pair(dmsT, qi[ib + 10], t[1 + ib + 10]);
-      const float p4 = pair(dmsT, qi[ib + 12], t[1 + ib + 12]);
-      if (ib + 4 < jend) CTH_DDMS_4CELL(4, s4, n2, n4, i, 0)
-      s0 = n4;
+      const float n2 = pair(dmsT, qi[ib + 6], t[1 + ib + 6]), nh2 = pairh(dmsT, qi[ib + 6], t[1 + ib + 6]);
+      const float n4 = pair(dmsT, qi[ib + 8], t[1 + ib + 8]), nh4 = pairh(dmsT, qi[ib + 8], t[1 + ib + 8]);
+      CTH_DDMS_4CELL(0, h0, s2, h2, s4, h4, i, 0)
+      const float p2 = pair(dmsT, qi[ib + 10], t[1 + ib + 10]), ph2 = pairh(dmsT, qi[ib + 10], t[1 + ib + 10]);
+      const float p4 = pair(dmsT, qi[ib + 12], t[1 + ib + 12]), ph4 = pairh(dmsT, qi[ib + 12], t[1 + ib + 12]);
+      if (ib + 4 < jend) CTH_DDMS_4CELL(4, h4, n2, nh2, n4, nh4, i, 0)
+      h0 = nh4;
       s2 = p2;
+      h2 = ph2;
       s4 = p4;
+      h4 = ph4;
     }
   }
   float s = 0.0f;
@@ -259,7 +273,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
   const int col = blockIdx.x * kCols + lane;
 
   // stage tables
-  for (int k = tid; k < 22 * 22; k += 256) L.dms[k] = tabs->dms[k];
+  for (int k = tid; k < 22 * 22; k += 256) {
+    L.dms[k] = tabs->dms[k];
+    L.dmsh[k] = 0.5f * tabs->dms[k];
+  }
   for (int k = tid; k < 467; k += 256) L.ss[k] = tabs->ss[k];
   for (int k = tid; k < tabs->cvcard; k += 256) L.cvvals[k] = tabs->cvvals[k];
   // stage 64 template lists (32 dwords each)
