@@ -44,15 +44,22 @@ __device__ __forceinline__ float ss_lookup(const float* ss, int row, int col) {
   return ss[col + 2];
 }
 
-// whether the segment [qb, qe] x [pb, pe] intersects the one whose packed
-// coordinates [QB QE PB PE] are c: byte-wise max of the begins <= min of the
-// ends on either axis (CuBatchScoreMatrix_ddms2s.cuh:141-153)
-__device__ __forceinline__ bool coords_intersect(uint32_t qb, uint32_t qe, uint32_t pb, uint32_t pe, uint32_t c) {
-  const uint32_t b0 = max(qb, c >> 24);
-  const uint32_t e0 = min(qe, (c >> 16) & 0xff);
-  const uint32_t b1 = max(pb, (c >> 8) & 0xff);
-  const uint32_t e1 = min(pe, c & 0xff);
-  return e0 >= b0 || e1 >= b1;
+// A segment's coordinates as two packed words: begins B = QB | PB << 16 and
+// ends E = (QE | PE << 16) + kEndBias.  Every index is below 256, so per half
+// E - B = 0x4000 + (end - begin) with no borrow between the halves, and bit
+// 14 of the half is set exactly when end >= begin.
+constexpr uint32_t kEndBias = 0x40004000u;
+// entries that intersect nothing: begins 255, ends 0
+constexpr uint32_t kNoBegin = 0x00ff00ffu, kNoEnd = kEndBias;
+
+// whether segment s (sb, se) intersects segment c (cb, ce): on the query or
+// the template axis the byte-wise max of the begins is <= the min of the
+// ends (CuBatchScoreMatrix_ddms2s.cuh:141-153), i.e. (begin <= end within
+// each segment) ce >= sb and se >= cb
+__device__ __forceinline__ bool coords_intersect(uint32_t sb, uint32_t se, uint32_t cb, uint32_t ce) {
+  uint32_t x = (ce - sb) & (se - cb);
+  asm("" : "+v"(x));  // keep the two-input form (the three-input logic op issues one per cycle)
+  return (x & kEndBias) != 0u;
 }
 
 // RegisterMaxDDMScore (CuBatchScoreMatrix_ddms2s.cuh:168-261): keep the
@@ -64,35 +71,33 @@ __device__ __forceinline__ bool coords_intersect(uint32_t qb, uint32_t qe, uint3
 //  m1 <= s < m0, s not intersecting m0: s becomes m1, the old m1 becomes m2
 //           unless it intersects s;
 //  m2 <= s < m1, s intersecting neither m0 nor m1: s becomes m2.
-// An entry whose value is 0 may keep stale coordinates: every intersection
-// test with it only decides between two zeros, and coordinates of list
-// indices (< 99) never intersect the initial 0xffffffff.  i0, i1: whether s
-// intersects the current m0, m1.  The register changes only if
-// register_changes(); otherwise every select below keeps the old value.
+// Only m0's and m1's coordinates are ever tested, so m2's are not kept.  An
+// entry whose value is 0 may keep stale coordinates: every intersection test
+// with it only decides between two zeros, and no segment intersects the
+// initial (kNoBegin, kNoEnd).  i0, i1: whether s intersects the current m0,
+// m1.  The register changes only if register_changes(); otherwise every
+// select below keeps the old value.
 __device__ __forceinline__ bool register_changes(float m0, float m1, float s, bool i0, bool i1) {
   return (m0 <= s) | (!i0 & ((m1 <= s) | !i1));
 }
-__device__ __forceinline__ void register_max(float& m0, float& m1, float& m2, uint32_t& c0, uint32_t& c1,
-                                             uint32_t& c2, float s, uint32_t sc, bool i0, bool i1) {
-  const bool a = m0 <= s, b1 = m1 <= s;
-  const bool b = !a & b1 & !i0;
-  const bool c = !a & !b1 & !i0 & !i1;
+__device__ __forceinline__ void register_max(float& m0, float& m1, float& m2, uint32_t& b0, uint32_t& e0,
+                                             uint32_t& b1, uint32_t& e1, float s, uint32_t sb, uint32_t se, bool i0,
+                                             bool i1) {
+  const bool a = m0 <= s, g1 = m1 <= s;
+  const bool b = !a & g1 & !i0;
+  const bool c = !a & !g1 & !i0 & !i1;
   const float ym1 = i1 ? 0.0f : m1;
   const float n1 = a ? (i0 ? ym1 : m0) : (b ? s : m1);
-  const uint32_t d1 = a ? (i0 ? c1 : c0) : (b ? sc : c1);
+  const uint32_t nb1 = a ? (i0 ? b1 : b0) : (b ? sb : b1);
+  const uint32_t ne1 = a ? (i0 ? e1 : e0) : (b ? se : e1);
   const float n2 = a ? (i0 ? 0.0f : ym1) : (b ? ym1 : (c ? s : m2));
-  const uint32_t d2 = (a | b) ? c1 : (c ? sc : c2);
   m0 = a ? s : m0;
-  c0 = a ? sc : c0;
+  b0 = a ? sb : b0;
+  e0 = a ? se : e0;
   m1 = n1;
-  c1 = d1;
+  b1 = nb1;
+  e1 = ne1;
   m2 = n2;
-  c2 = d2;
-}
-
-// [QB QE PB PE] packed; each is a list index (< CTH_MAXDST), so no masking
-__device__ __forceinline__ uint32_t mkc(int qb, int qe, int pb, int pe) {
-  return ((uint32_t)qb << 24) | ((uint32_t)qe << 16) | ((uint32_t)pb << 8) | (uint32_t)pe;
 }
 
 // CalcDDMS2ScoreSMEMh2<2,true,2> (cusco/CuBatchScoreMatrix_ddms2s_h2.cuh:435-500):
@@ -157,17 +162,21 @@ __device__ __forceinline__ uint32_t mkc(int qb, int qe, int pb, int pe) {
     CTH_DDMS_CELL(A4, dd4, cd4)                                               \
     const float scom = fmaxf(fmaxf(cd1, cd2), fmaxf(cd3, cd4));               \
     if (scom >= m2f) {                                                        \
-      int k = 4;                                                              \
+      uint32_t kk = 4u * 0x10001u;                                            \
       float dk = dd4;                                                         \
-      k = cd3 == scom ? 3 : k; dk = cd3 == scom ? dd3 : dk;                   \
-      k = cd2 == scom ? 2 : k; dk = cd2 == scom ? dd2 : dk;                   \
-      k = cd1 == scom ? 1 : k; dk = cd1 == scom ? dd1 : dk;                   \
-      const int be = ib + (K0) + k, bb = be + (int)dk;                        \
-      const uint32_t qb = bb + (QOFF), qe = be + (QOFF), pb = bb + (POFF), pe = be + (POFF); \
-      const bool i0 = coords_intersect(qb, qe, pb, pe, c0);                   \
-      const bool i1 = coords_intersect(qb, qe, pb, pe, c1);                   \
+      kk = cd3 == scom ? 3u * 0x10001u : kk; dk = cd3 == scom ? dd3 : dk;     \
+      kk = cd2 == scom ? 2u * 0x10001u : kk; dk = cd2 == scom ? dd2 : dk;     \
+      kk = cd1 == scom ? 0x10001u : kk; dk = cd1 == scom ? dd1 : dk;          \
+      /* end (ib + K0 + k) and begin (end + dk) on both axes, packed */       \
+      uint32_t se = kk + ((uint32_t)(ib + (K0)) * 0x10001u + (uint32_t)(QOFF) + \
+                          ((uint32_t)(POFF) << 16) + kEndBias);               \
+      asm("" : "+v"(se));                                                     \
+      uint32_t sb = (se - kEndBias) + (uint32_t)(int)(dk * 65537.0f);         \
+      asm("" : "+v"(sb));                                                     \
+      const bool i0 = coords_intersect(sb, se, cb0, ce0);                     \
+      const bool i1 = coords_intersect(sb, se, cb1, ce1);                     \
       if (register_changes(m0, m1, scom, i0, i1)) {                           \
-        register_max(m0, m1, m2, c0, c1, c2, scom, mkc(qb, qe, pb, pe), i0, i1); \
+        register_max(m0, m1, m2, cb0, ce0, cb1, ce1, scom, sb, se, i0, i1);   \
         m2f = __builtin_amdgcn_fmed3f(m2, FLT_MIN, FLT_MAX);                  \
       }                                                                       \
     }                                                                         \
@@ -191,7 +200,7 @@ __device__ __forceinline__ float ddms_cell(const float* __restrict__ dmsT, const
   const int len1 = q[0], len2 = t[0];
   if (len1 < CTH_DSTSEGM || len2 < CTH_DSTSEGM) return 0.0f;
   float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f, m2f = FLT_MIN;
-  uint32_t c0 = 0xffffffffu, c1 = 0xffffffffu, c2 = 0xffffffffu;
+  uint32_t cb0 = kNoBegin, ce0 = kNoEnd, cb1 = kNoBegin, ce1 = kNoEnd;
   // 2^126 in a vector register: as a scalar operand it would keep the
   // multiply off the second issue slot
   float big;
