@@ -75,29 +75,36 @@ __device__ __forceinline__ bool coords_intersect(uint32_t sb, uint32_t se, uint3
 // entry whose value is 0 may keep stale coordinates: every intersection test
 // with it only decides between two zeros, and no segment intersects the
 // initial (kNoBegin, kNoEnd).  i0, i1: whether s intersects the current m0,
-// m1.  The register changes only if register_changes(); otherwise every
-// select below keeps the old value.
+// m1.  The register changes only if register_changes(); register_max is
+// called only then.
 __device__ __forceinline__ bool register_changes(float m0, float m1, float s, bool i0, bool i1) {
   return (m0 <= s) | (!i0 & ((m1 <= s) | !i1));
 }
 __device__ __forceinline__ void register_max(float& m0, float& m1, float& m2, uint32_t& b0, uint32_t& e0,
                                              uint32_t& b1, uint32_t& e1, float s, uint32_t sb, uint32_t se, bool i0,
                                              bool i1) {
-  const bool a = m0 <= s, g1 = m1 <= s;
-  const bool b = !a & g1 & !i0;
-  const bool c = !a & !g1 & !i0 & !i1;
-  const float ym1 = i1 ? 0.0f : m1;
-  const float n1 = a ? (i0 ? ym1 : m0) : (b ? s : m1);
-  const uint32_t nb1 = a ? (i0 ? b1 : b0) : (b ? sb : b1);
-  const uint32_t ne1 = a ? (i0 ? e1 : e0) : (b ? se : e1);
-  const float n2 = a ? (i0 ? 0.0f : ym1) : (b ? ym1 : (c ? s : m2));
-  m0 = a ? s : m0;
-  b0 = a ? sb : b0;
-  e0 = a ? se : e0;
-  m1 = n1;
-  b1 = nb1;
-  e1 = ne1;
-  m2 = n2;
+  // one block per case, so that a wavefront runs only the cases its lanes
+  // have (most often only the first: the best segment growing)
+  if (m0 <= s) {
+    // s becomes m0
+    const float ym1 = i1 ? 0.0f : m1;
+    m2 = i0 ? 0.0f : ym1;
+    m1 = i0 ? ym1 : m0;
+    b1 = i0 ? b1 : b0;
+    e1 = i0 ? e1 : e0;
+    m0 = s;
+    b0 = sb;
+    e0 = se;
+  } else if (m1 <= s) {
+    // s becomes m1 (s does not intersect m0 when the register changes)
+    m2 = i1 ? 0.0f : m1;
+    m1 = s;
+    b1 = sb;
+    e1 = se;
+  } else {
+    // s becomes m2 (s intersects neither m0 nor m1 when the register changes)
+    m2 = s;
+  }
 }
 
 // CalcDDMS2ScoreSMEMh2<2,true,2> (cusco/CuBatchScoreMatrix_ddms2s_h2.cuh:435-500):
