@@ -85,12 +85,13 @@ __global__ __launch_bounds__(64) void k_dp(cth_dpset q, int qp, cth_dpset db, in
     float dt[CTH_NTRN];
 #pragma unroll
     for (int i = 0; i < CTH_NTRN; i++) dt[i] = kLog0;
-    const int nsteps = lt + 95;
-    for (int t = 0; t < nsteps; t++) {
-      const int c = t - lane - 32;
-      // template transitions: lane 0 loads column t-32, the others take lane-1's
-      const int cu = t - 32;
-      float nt[CTH_NTRN];
+    // Loads run ahead of their use (the step itself is a few hundred cycles
+    // of dependent arithmetic): the template column's transitions and the
+    // previous stripe's bottom row one step ahead, this lane's score two
+    // steps ahead.  Step t uses template column t - 32 and lane column
+    // t - lane - 32.
+    float ntn[CTH_NTRN], bn[5];
+    auto load_col = [&](int cu, float* nt, float* b) {
       if (0 <= cu && cu < lt) {
         nt[CTH_MM] = db.trn[CTH_MM * db.ldtr + tb + cu];
         nt[CTH_MI] = db.trn[CTH_MI * db.ldtr + tb + cu + 1];
@@ -103,24 +104,47 @@ __global__ __launch_bounds__(64) void k_dp(cth_dpset q, int qp, cth_dpset db, in
 #pragma unroll
         for (int i = 0; i < CTH_NTRN; i++) nt[i] = kLog0;
       }
+      if (s > 0 && 0 <= cu && cu < lt) {
+        b[0] = bot[0 * ltmax + cu]; b[1] = bot[1 * ltmax + cu]; b[2] = bot[2 * ltmax + cu];
+        b[3] = bot[3 * ltmax + cu]; b[4] = bot[4 * ltmax + cu];
+      } else {
+        b[0] = b[1] = b[2] = b[3] = b[4] = 0.0f;
+      }
+    };
+    // the raw score at lane column c (0 outside the profile; the valid-area
+    // test is applied where it is used)
+    const float* __restrict__ srow = scores + (int64_t)y * ldsc + colp;
+    auto load_sc = [&](int c) { return (rowok && 0 <= c && c < lt) ? srow[c] : 0.0f; };
+    load_col(-32, ntn, bn);
+    float sc1 = load_sc(-lane - 32), sc2 = load_sc(-lane - 31);
+    const int nsteps = lt + 95;
+    for (int t = 0; t < nsteps; t++) {
+      const int c = t - lane - 32;
+      const int cu = t - 32;
+      // this step's column data, then the loads for the steps after
+      float nt[CTH_NTRN], bcu[5];
+#pragma unroll
+      for (int i = 0; i < CTH_NTRN; i++) nt[i] = ntn[i];
+#pragma unroll
+      for (int i = 0; i < 5; i++) bcu[i] = bn[i];
+      const float scv = sc1;
+      sc1 = sc2;
+      sc2 = load_sc(c + 2);
+      load_col(cu + 1, ntn, bn);
+      // template transitions: lane 0 takes column t-32, the others lane-1's
 #pragma unroll
       for (int i = 0; i < CTH_NTRN; i++) dt[i] = dpp_shr1(nt[i], dt[i]);
       // up state: lane 0 reads the previous stripe's bottom row (zero for the
       // first stripe and for negative columns), the others take lane-1's
-      float b0 = 0, b1 = 0, b2 = 0, b3 = 0, b4 = 0;
-      if (s > 0 && 0 <= cu && cu < lt) {
-        b0 = bot[0 * ltmax + cu]; b1 = bot[1 * ltmax + cu]; b2 = bot[2 * ltmax + cu];
-        b3 = bot[3 * ltmax + cu]; b4 = bot[4 * ltmax + cu];
-      }
-      float uMM = dpp_shr1(b0, cMM), uMI = dpp_shr1(b1, cMI), uIM = dpp_shr1(b2, cIM);
-      float uDN = dpp_shr1(b3, cDN), uND = dpp_shr1(b4, cND);
+      float uMM = dpp_shr1(bcu[0], cMM), uMI = dpp_shr1(bcu[1], cMI), uIM = dpp_shr1(bcu[2], cIM);
+      float uDN = dpp_shr1(bcu[3], cDN), uND = dpp_shr1(bcu[4], cND);
       if ((lane & 31) == 0 && c < 0) { uMM = uMI = uIM = uDN = uND = 0.0f; }
       if ((lane & 31) == 31 && c == 0) {  // thread 0 of the block at x = 0: zero left/diagonal
         cMM = cMI = cIM = cDN = cND = 0.0f;
         pMM = pMI = pIM = pDN = pND = 0.0f;
       }
       if (rowok && c0 <= c && c < lt) {
-        const float S = in_valid_area(lq, lt, edge, y, c) ? scores[(int64_t)y * ldsc + colp + c] : CTH_OUTSCORE;
+        const float S = in_valid_area(lq, lt, edge, y, c) ? scv : CTH_OUTSCORE;
         float v1, v2, m;
         int bk;
         v1 = pMI + qt[CTH_MM] + dt[CTH_IM];
