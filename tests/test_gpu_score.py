@@ -35,3 +35,30 @@ def test_score_matrix_subrange(ctx, oracle, sets):
     osc, omd = oracle.score_matrix(q, 1, db, 7, 19)
     assert (sc.view(np.int32) == osc.view(np.int32)).all()
     assert (md.view(np.int32) == omd.view(np.int32)).all()
+
+
+def test_score_matrix_adversarial_distance_lists(ctx, oracle):
+    """Every position's distance list replaced by a hand-shaped one
+    (tests/_ddms_lists.py: constant, near-zero, alternating, run-structured,
+    missing-heavy, sorted; lengths 0..99 with 5..9 over-represented): the
+    segment restarts, length test, first-maximum selection and top-3
+    register of the DDMS term, bit for bit against the oracle."""
+    from tests import _ddms_lists as dl
+    q = cother_amd.synth(2, 0, lmin=60, lmax=130, dist_mode=0, sort_desc=0, seed=141)
+    db = cother_amd.synth(30, 0, lmin=8, lmax=160, dist_mode=1, sort_desc=1, seed=142)
+    rng = np.random.default_rng(143)
+    dl.fill(q, rng)
+    dl.fill(db, rng)
+    # a few templates carry the query's own lists: long top segments
+    qdd, ddd = q.field("dd"), db.field("dd")
+    n = min(int(q.lens()[0]), int(db.lens()[1]))
+    ddd[int(db.dists()[1]):int(db.dists()[1]) + n] = qdd[:n]
+    ctx.upload_queries(q)
+    ctx.upload_db(db)
+    for qi in range(q.nprof):
+        print(f"adversarial lists: query {qi}", flush=True)
+        sc, md = ctx.score_block(qi, 0, db.nprof)
+        osc, omd = oracle.score_matrix(q, qi, db, 0, db.nprof)
+        bad = np.argwhere(sc.view(np.int32) != osc.view(np.int32))
+        assert bad.size == 0, f"query {qi}: {len(bad)} cells differ, first {bad[:5].tolist()}"
+        assert (md.view(np.int32) == omd.view(np.int32)).all()

@@ -151,6 +151,34 @@ def test_ddms_matches_reference_cpu_ddms(oracle):
     assert nchk == 3000
 
 
+@pytest.mark.skipif(_oracle.load_ref_ddms() is None, reason="reference CPU DDMS not built (oracle/_ref)")
+def test_ddms_matches_reference_on_adversarial_lists(oracle):
+    """The same comparison on hand-shaped lists (tests/_ddms_lists.py):
+    constant, near-zero, alternating, run-structured, missing-heavy and
+    sorted lists, including pairs of identical lists."""
+    from tests import _ddms_lists as dl
+    ref = _oracle.load_ref_ddms()
+    tab = _ref_style_table()
+    rng = np.random.default_rng(17)
+    for it in range(2000):
+        n1, n2 = int(rng.integers(24, 100)), int(rng.integers(24, 100))
+        k1, k2 = dl.KINDS[it % len(dl.KINDS)], dl.KINDS[(it // len(dl.KINDS)) % len(dl.KINDS)]
+        q, t = dl.record(rng, k1, n1), dl.record(rng, k2, n2)
+        if it % 5 == 0:
+            t = q.copy()
+        qi = np.zeros(128 + 8, np.int32)
+        ti = np.zeros(128 + 8, np.int32)
+        qi[:128], ti[:128] = q, t
+        qi[128:], ti[128:] = 255, 255
+        want = ref.ref_ddms_calculate(qi.ctypes.data, ti.ctypes.data)
+        _, m, _ = oracle.ddms_tab(tab, q, t)
+        s = np.float32(0)
+        for k in range(3):
+            s = np.float32(s + m[k])
+        got = np.float32(np.rint(np.float32(s * np.float32(16.0))))
+        assert got == want, (it, k1, k2, n1, n2, got, want)
+
+
 def test_ddms_short_lists_score_zero(oracle):
     rng = np.random.default_rng(3)
     q = _random_list(rng, 5)
