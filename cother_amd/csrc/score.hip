@@ -57,9 +57,7 @@ constexpr uint32_t kNoBegin = 0x00ff00ffu, kNoEnd = kEndBias;
 // ends (CuBatchScoreMatrix_ddms2s.cuh:141-153), i.e. (begin <= end within
 // each segment) ce >= sb and se >= cb
 __device__ __forceinline__ bool coords_intersect(uint32_t sb, uint32_t se, uint32_t cb, uint32_t ce) {
-  uint32_t x = (ce - sb) & (se - cb);
-  asm("" : "+v"(x));  // keep the two-input form (the three-input logic op issues one per cycle)
-  return (x & kEndBias) != 0u;
+  return ((ce - sb) & (se - cb) & kEndBias) != 0u;
 }
 
 // RegisterMaxDDMScore (CuBatchScoreMatrix_ddms2s.cuh:168-261): keep the
