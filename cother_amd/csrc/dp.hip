@@ -80,6 +80,10 @@ __global__ __launch_bounds__(64) void k_dp(cth_dpset q, int qp, cth_dpset db, in
 #pragma unroll
       for (int i = 0; i < CTH_NTRN; i++) qt[i] = kLog0;
     }
+    // wait for the query transitions here, once per stripe: a wait inside
+    // the step loop would also wait for the prefetches issued in it
+#pragma unroll
+    for (int i = 0; i < CTH_NTRN; i++) asm volatile("" ::"v"(qt[i]));
     float cMM = 0, cMI = 0, cIM = 0, cDN = 0, cND = 0;     // state of this lane's last column
     float pMM = 0, pMI = 0, pIM = 0, pDN = 0, pND = 0;     // up-state received last step (= diagonal now)
     float dt[CTH_NTRN];
