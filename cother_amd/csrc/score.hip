@@ -13,10 +13,11 @@
 // columns; wave w owns query row w and lane l column l, so the query's
 // distance list is uniform across a wavefront and each lane walks its own
 // template list.  The 64 template lists (132-byte padded rows, conflict-free
-// byte reads), the 4 query lists and the 22x22 pair-score table are staged in
-// LDS.  The per-cell DDMS dynamic programme (the dominant cost, ~2200 cell
-// updates per score cell for full lists) runs entirely from LDS and
-// registers.
+// byte reads) and the 22x22 pair-score table with its halved copy are staged
+// in LDS; a wavefront's query list is read by scalar loads, as pair-table
+// byte offsets.  The per-cell DDMS dynamic programme (the dominant cost,
+// ~2200 cell updates per score cell for full lists) runs from LDS, scalar
+// registers and vector registers.
 #include <float.h>
 
 #include <hip/hip_runtime.h>
@@ -151,13 +152,16 @@ __device__ __forceinline__ void register_max(float& m0, float& m1, float& m2, ui
 // scores A2, A4 (cells K0+2 and K0+4) and the halves H0, H2, H4 of those of
 // cells K0 (the previous step's last cell), K0+2 and K0+4: the reference's
 // interpolated sum 0.5f * (a + b) is h(a) + h(b), halving being exact and
-// commuting with the rounding of the sum; the begin and end of the step's best are recovered from the cells'
-// candidates only when it can be registered.  Two levels: the step's best is
-// a candidate when it reaches max(m2, FLT_MIN); a candidate changes the
-// register only when it passes the intersection tests (most candidates
-// intersect m0 or m1 while scoring below them: segments of neighbouring
-// diagonals through the same region), so the selects run for a wavefront
-// only when one of its lanes really changes its register.
+// commuting with the rounding of the sum.  The begin and end of the step's
+// best are recovered from the cells' candidates only when it can be
+// registered.  Two levels: the step's best is a candidate when it reaches
+// max(m2, FLT_MIN); a candidate changes the register only when it passes the
+// intersection tests (most candidates intersect m0 or m1 while scoring below
+// them: segments of neighbouring diagonals through the same region), so the
+// update runs for a wavefront only when one of its lanes really changes its
+// register.  The empty asm statements keep the candidate's packed words in
+// registers: without them the compiler folds their sums into each test as
+// scalar-operand adds, which issue one per cycle.
 #define CTH_DDMS_4CELL(K0, H0, A2, H2, A4, H4, QOFF, POFF)                    \
   {                                                                           \
     float cd1, cd2, cd3, cd4, dd1, dd2, dd3, dd4;                             \
