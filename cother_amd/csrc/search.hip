@@ -145,8 +145,6 @@ extern "C" int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt
       while (p1 < nprof && (db.h_dist[p1] + db.h_len[p1] - col0 <= chunkcols || p1 == p0)) p1++;
       const int64_t ncols = db.h_dist[p1 - 1] + db.h_len[p1 - 1] - col0;
       const int64_t ldsc = (ncols + 63) / 64 * 64;
-      int ltmax = 0;
-      for (int p = p0; p < p1; p++) ltmax = std::max(ltmax, db.h_len[p]);
       const size_t need = sizeof(float) * (size_t)lq * ldsc;
       if (cth_ctx_reserve(c, (void**)&c->scores, &c->cap_scores, need)) return -1;
       if (cth_ctx_reserve(c, (void**)&c->mods, &c->cap_mods, need)) return -1;
@@ -160,10 +158,19 @@ extern "C" int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt
       // phase 1 in two passes: the DP of every profile without traceback,
       // then, for the profiles whose maximum exceeds the score threshold (the
       // only ones FinalizeDPPro traces back, cudp/CuBatchDP_final.cu:144),
-      // the same DP with traceback and the finalisation
-      if (cth_launch_dp_list(&c->q.d, qi, &db.d, p0, nullptr, p1 - p0, ltmax, col0, ldsc, c->scores, nullptr,
-                             scorethld, 0, (cth_dpres*)c->dpout, c->st))
-        return -1;
+      // the same DP with traceback and the finalisation.  The first pass runs
+      // in groups of consecutive profiles, each launch reserving LDS for its
+      // own longest profile, so that groups of shorter profiles (databases
+      // are sorted by length) run more wavefronts per CU.
+      constexpr int kDpGroup = 2048;
+      for (int a = p0; a < p1; a += kDpGroup) {
+        const int b = std::min(p1, a + kDpGroup);
+        int lg = 0;
+        for (int p = a; p < b; p++) lg = std::max(lg, db.h_len[p]);
+        if (cth_launch_dp_list(&c->q.d, qi, &db.d, a, nullptr, b - a, lg, col0, ldsc, c->scores, nullptr, scorethld,
+                               0, (cth_dpres*)c->dpout + (a - p0), c->st))
+          return -1;
+      }
       if (timing) CTH_HIP_CHECK(hipEventRecord(c->ev[2], c->st));
       std::vector<cth_dpres> res(p1 - p0);
       CTH_HIP_CHECK(hipMemcpyAsync(res.data(), c->dpout, sizeof(cth_dpres) * res.size(), hipMemcpyDeviceToHost,
