@@ -121,7 +121,9 @@ extern "C" int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt
   const cth_devset& db = c->db;
   const int nprof = db.d.nprof;
   const uint64_t dbsize = (uint64_t)db.d.npos;
-  const size_t budget = opt->mem_budget ? opt->mem_budget : ((size_t)16 << 30);
+  // 64 GiB by default: a quarter of the device's HBM, so that long queries
+  // run in few, wide chunks (more template profiles per DP launch)
+  const size_t budget = opt->mem_budget ? opt->mem_budget : ((size_t)64 << 30);
   const bool timing = opt->timing != 0;
   if (timing && !c->has_ev) {
     for (hipEvent_t& e : c->ev) CTH_HIP_CHECK(hipEventCreate(&e));
@@ -136,8 +138,12 @@ extern "C" int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt
     score_threshold(opt->eval, (uint64_t)lq, dbsize, &scorethld, &logethld);
     const double sspace = (double)length_adjustment((uint64_t)lq, dbsize, (uint64_t)nprof);
     // chunk columns so that scores + modscores + traceback (+ MAP posteriors) fit the budget
+    // (and at most 6M columns: the phase-1 DP reads 64 score rows per step,
+    // which is slower when the rows are much further apart)
     const size_t bytes_per_cell = opt->do_map ? 13 : 9;
-    const int64_t chunkcols = std::max<int64_t>(4096, (int64_t)(budget / ((size_t)lq * bytes_per_cell)));
+    constexpr int64_t kMaxChunkCols = 6 << 20;
+    const int64_t chunkcols =
+        std::max<int64_t>(4096, std::min<int64_t>(kMaxChunkCols, (int64_t)(budget / ((size_t)lq * bytes_per_cell))));
     const int qh0 = nh;
     for (int p0 = 0; p0 < nprof;) {
       int p1 = p0;

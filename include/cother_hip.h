@@ -169,7 +169,7 @@ typedef struct cth_search_opts {
   int ssemodel;       /* statistical significance model (SSEMODEL, default 2) */
   int do_map;         /* realign passed hits by MAP (MAPALN, default 1) */
   int timing;         /* record HIP events around each stage on the context's stream */
-  uint64_t mem_budget;/* bytes for the per-chunk score/traceback buffers (0: 16 GiB) */
+  uint64_t mem_budget;/* bytes for the per-chunk score/traceback buffers (0: 64 GiB) */
   float minpp;        /* MAP posterior threshold (MINPP; 0: the default 0.28) */
   int contraction;    /* log-odds term: 0 FP32 as the reference (default); 1 BF16, 2 FP16 operands on
                          the matrix cores with FP32 accumulation (no reference counterpart) */
