@@ -63,3 +63,25 @@ def test_search_at_the_cpu_runnable_size(ctx, oracle):
     osc, omd = oracle.score_matrix(q, 0, db, 0, 60)
     assert (sc.view(np.int32) == osc.view(np.int32)).all()
     assert (md.view(np.int32) == omd.view(np.int32)).all()
+
+
+def test_search_over_several_dp_groups(ctx, oracle):
+    """More than 2048 templates in one chunk: the first phase-1 pass runs in
+    groups of 2048 profiles (each launch sized for its own longest profile,
+    records written at the group's offset); 4500 short templates, sorted by
+    length as databases are, give three groups.  Hits (many, at a permissive
+    E-value) equal the oracle's."""
+    q = cother_amd.synth(1, 0, lmin=40, lmax=40, dist_mode=0, sort_desc=0, seed=61)
+    db = cother_amd.synth(4500, 0, lmin=8, lmax=70, dist_mode=1, sort_desc=1, seed=62)
+    ctx.upload_queries(q)
+    ctx.upload_db(db)
+    print(f"search: Lq=40 vs {db.nprof} templates", flush=True)
+    hits, nh, st = ctx.search(eval=1e5)
+    want = oracle.search_query(q, 0, db, eval_=1e5)
+    print(f"{len(hits)} hits (oracle {len(want)})", flush=True)
+    assert len(hits) == len(want) >= 50
+    assert hits["profile"].max() >= 4096  # hits from the third group
+    for f in FIELDS:
+        assert (hits[f] == want[f]).all(), f
+    for f in STATS:
+        np.testing.assert_allclose(hits[f], want[f], rtol=2e-6, atol=1e-30, err_msg=f)
