@@ -44,7 +44,8 @@ class SearchOpts(ctypes.Structure):
     """Mirror of cth_search_opts (include/cother_hip.h)."""
     _fields_ = [("eval", ctypes.c_float), ("ssemodel", ctypes.c_int), ("do_map", ctypes.c_int),
                 ("timing", ctypes.c_int), ("mem_budget", ctypes.c_uint64), ("minpp", ctypes.c_float),
-                ("contraction", ctypes.c_int), ("alnbuf", ctypes.c_void_p), ("alncap", ctypes.c_uint64)]
+                ("contraction", ctypes.c_int), ("alnbuf", ctypes.c_void_p), ("alncap", ctypes.c_uint64),
+                ("dbsize", ctypes.c_uint64), ("ndb", ctypes.c_uint64)]
 
 
 class SearchStats(ctypes.Structure):
@@ -478,8 +479,11 @@ class Context:
         return res
 
     def search(self, q0=0, q1=None, eval=10.0, ssemodel=2, do_map=True, timing=False, mem_budget=0,
-               maxhits=None, minpp=MINPP, alignments=False, contraction=0):
+               maxhits=None, minpp=MINPP, alignments=False, contraction=0, dbsize=0, ndb=0):
         """Search queries [q0,q1) against the resident database.
+
+        dbsize, ndb: positions and profiles of the whole database when the resident set is one
+        part of it (0, 0: the resident set's own).
 
         Returns (hits HIT array grouped by query, per-query hit counts, SearchStats), and with
         alignments=True also a list of five-row alignments (query, middle, target, query SS,
@@ -497,7 +501,7 @@ class Context:
             abuf = ctypes.create_string_buffer(cap)
         opts = SearchOpts(eval, ssemodel, int(do_map), int(timing), mem_budget, minpp, contraction,
                           ctypes.cast(abuf, ctypes.c_void_p) if abuf is not None else None,
-                          len(abuf) if abuf is not None else 0)
+                          len(abuf) if abuf is not None else 0, dbsize, ndb)
         _check(lib().cth_search(self.h, q0, q1, ctypes.byref(opts), hits.ctypes.data, maxhits, nh.ctypes.data,
                                 ctypes.byref(st)), "cth_search")
         hits = hits[:int(nh.sum())]

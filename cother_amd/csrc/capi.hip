@@ -253,6 +253,8 @@ extern "C" int cth_launch_stats(const cth_dpset* q, int qp, const cth_dpset* db,
 // lambda, K}.  plist and stats must hold p1-p0 entries.
 extern "C" int cth_phase2_block(cth_ctx* c, int qidx, int p0, int p1, float scorethld, double searchspace,
                                 int ssemodel, void* res, int* plist, int* npass, double* stats) {
+  // SSEMODEL takes 0..2 (libpro/srcpro/MOptions.cpp:123)
+  if (ssemodel < 0 || ssemodel > 2) { cth_set_error("phase2_block: ssemodel %d (0, 1 or 2)", ssemodel); return -1; }
   if (cth_phase1_block(c, qidx, p0, p1, scorethld, res, nullptr)) return -1;
   const int n = p1 - p0;
   const cth_dpres* r = (const cth_dpres*)res;

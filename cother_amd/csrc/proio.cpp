@@ -413,12 +413,29 @@ extern "C" cth_pset* cth_read_binary_db(const char* path) {
          sCVnorm2, sDst, sSS = sDst + 100, sHDPprb = sSS + 3, sHDPind, sAA, sSSstate, sDescEnd, sDesc };
   static_assert(sDesc + 1 == kV22Sections, "address table size");
   const size_t npos = (size_t)dbsize;
-  auto need = [&](int s, size_t bytes) { return addr[s] + bytes <= buf.size(); };
-  if (!need(sLen, 4 * nprof) || !need(sSSstate, npos)) { cth_set_error("%s: truncated sections", path); return nullptr; }
+  // every section must lie inside the file (sizes checked before they are
+  // multiplied, offsets before they are added)
+  if (nprof > buf.size() / 4 || npos > buf.size() / 4 || nprof > (size_t)INT32_MAX) {
+    cth_set_error("%s: header counts %zu profiles, %zu positions", path, nprof, npos);
+    return nullptr;
+  }
+  auto need = [&](int s, size_t bytes) { return addr[s] <= buf.size() && bytes <= buf.size() - addr[s]; };
+  const size_t ntr = npos + nprof;
+  bool inside = need(sLen, 4 * nprof) && need(sENO, 4 * nprof) && need(sAA, npos) && need(sSSstate, npos);
+  for (int r = 0; r < CTH_NAA; r++) inside = inside && need(sBkg + r, 4 * nprof) && need(sTrg + r, 4 * npos);
+  for (int t = 0; t < CTH_NTRN; t++) inside = inside && need(sTrn + t, 4 * ntr);
+  for (int k = 0; k < CTH_NCV; k++) inside = inside && need(sCV + k, 4 * npos);
+  inside = inside && need(sCVprior, 4 * npos) && need(sCVnorm2, 4 * npos);
+  for (int k = 0; k <= kMaxDst; k++) inside = inside && need(sDst + k, 4 * npos);
+  for (int q = 0; q < 3; q++) inside = inside && need(sSS + q, 4 * npos);
+  if (!inside) { cth_set_error("%s: truncated sections", path); return nullptr; }
   std::vector<int32_t> lens(nprof);
   memcpy(lens.data(), buf.data() + addr[sLen], 4 * nprof);
   size_t tot = 0;
-  for (int32_t l : lens) tot += (size_t)l;
+  for (int32_t l : lens) {
+    if (l < 1) { cth_set_error("%s: profile length %d", path, (int)l); return nullptr; }
+    tot += (size_t)l;
+  }
   if (tot != npos) { cth_set_error("%s: lengths sum to %zu, header says %zu", path, tot, npos); return nullptr; }
   cth_pset* o = cth_pset_alloc((int)nprof, lens.data());
   if (!o) return nullptr;
@@ -427,7 +444,6 @@ extern "C" cth_pset* cth_read_binary_db(const char* path) {
     memcpy(&v, buf.data() + addr[s] + 4 * k, 4);
     return v;
   };
-  const size_t ntr = npos + nprof;
   for (size_t p = 0; p < nprof; p++) {
     o->eno[p] = f32(sENO, p);
     for (int r = 0; r < CTH_NAA; r++) o->bkg[p * CTH_NAA + r] = f32(sBkg + r, p);

@@ -112,15 +112,37 @@ extern "C" void cth_search_space(uint64_t qlen, uint64_t dbsize, uint64_t ndb, u
   if (deltalen) *deltalen = dl;
 }
 
+// Option checks shared by every entry point that takes cth_search_opts.
+extern "C" int cth_search_opts_check(const cth_search_opts* opt, const char* who) {
+  if (!opt) { cth_set_error("%s: no options", who); return -1; }
+  if (opt->contraction < 0 || opt->contraction > 2) { cth_set_error("%s: contraction %d", who, opt->contraction); return -1; }
+  // SSEMODEL takes 0..2 (libpro/srcpro/MOptions.cpp:123)
+  if (opt->ssemodel < 0 || opt->ssemodel > 2) {
+    cth_set_error("%s: ssemodel %d (0, 1 or 2)", who, opt->ssemodel);
+    return -1;
+  }
+  if ((opt->dbsize == 0) != (opt->ndb == 0)) {
+    cth_set_error("%s: dbsize and ndb are given together (both 0: the resident set's)", who);
+    return -1;
+  }
+  if (opt->ndb > opt->dbsize) { cth_set_error("%s: ndb %llu > dbsize %llu", who, (unsigned long long)opt->ndb,
+                                              (unsigned long long)opt->dbsize); return -1; }
+  return 0;
+}
+
 extern "C" int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt, cth_hit* hits, int maxhits,
                           int* nhits, cth_search_stats* stats) {
   if (!c->has_q || !c->has_db) { cth_set_error("search: queries or database not uploaded"); return -1; }
   if (q0 < 0 || q1 > c->q.d.nprof || q0 >= q1) { cth_set_error("search: bad query range [%d,%d)", q0, q1); return -1; }
-  if (opt->contraction < 0 || opt->contraction > 2) { cth_set_error("search: contraction %d", opt->contraction); return -1; }
+  if (cth_search_opts_check(opt, "search")) return -1;
   CTH_HIP_CHECK(hipSetDevice(c->device));
   const cth_devset& db = c->db;
   const int nprof = db.d.nprof;
-  const uint64_t dbsize = (uint64_t)db.d.npos;
+  // the database size and count the statistics use: the whole database's
+  // when the resident set is one part of it (CuBatchProcessing::SetDbDetails,
+  // cupro/CuBatchProcessing.cuh:73, set per worker at cualn/DevCommThread.cpp:165)
+  const uint64_t dbsize = opt->dbsize ? opt->dbsize : (uint64_t)db.d.npos;
+  const uint64_t ndbent = opt->ndb ? opt->ndb : (uint64_t)nprof;
   // 64 GiB by default: a quarter of the device's HBM, so that long queries
   // run in few, wide chunks (more template profiles per DP launch)
   const size_t budget = opt->mem_budget ? opt->mem_budget : ((size_t)64 << 30);
@@ -136,7 +158,7 @@ extern "C" int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt
     const int lq = c->q.h_len[qi];
     float scorethld, logethld;
     score_threshold(opt->eval, (uint64_t)lq, dbsize, &scorethld, &logethld);
-    const double sspace = (double)length_adjustment((uint64_t)lq, dbsize, (uint64_t)nprof);
+    const double sspace = (double)length_adjustment((uint64_t)lq, dbsize, ndbent);
     // chunk columns so that scores + modscores + traceback (+ MAP posteriors) fit the budget
     // (and at most 6M columns: the phase-1 DP reads 64 score rows per step,
     // which is slower when the rows are much further apart)

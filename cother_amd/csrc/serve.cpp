@@ -43,6 +43,7 @@
 
 void cth_set_error(const char* fmt, ...);
 extern "C" int cth_ctx_create_shared(cth_ctx* parent, cth_ctx** out);
+extern "C" int cth_search_opts_check(const cth_search_opts* opt, const char* who);
 extern "C" void cth_ctx_trim(cth_ctx* c, size_t keep);
 extern "C" int cth_ctx_shape(const cth_ctx* c, int* nqueries, const int32_t** qlens, int* ndb, int64_t* dbpos);
 
@@ -163,6 +164,7 @@ extern "C" int cth_server_create(cth_ctx* parent, const cth_server_opts* o, cth_
     cth_set_error("server: invalid arguments");
     return -1;
   }
+  if (cth_search_opts_check(&o->search, "server")) return -1;
   std::unique_ptr<cth_server> s(new cth_server);
   s->parent = parent;
   s->opt = *o;

@@ -205,6 +205,9 @@ __device__ double lgammaincq(double a, double x) {
 
 __device__ double sig2(double v) { return 2.0 / (1.0 + exp(-v)) - 1.0; }
 
+// one network (cuss/CuBatchSS_NN.cuh:37-148): hidden unit h has bias w[5h]
+// and input weights w[5h+1..5h+4], the output bias w[20] and unit weights
+// w[21..24]; activation 2/(1+e^-x) - 1
 __device__ double nn_out(const double in[4], const float* w, double factor) {
   double acc = w[20];
   for (int h = 0; h < 4; h++) {
@@ -215,7 +218,15 @@ __device__ double nn_out(const double in[4], const float* w, double factor) {
   return sig2(acc) * factor;
 }
 
-// SS18 significance: out = [lmbdest, kest, logE, logEpA]
+// constants of cuss/CuBatchSS_SS18.cuh:17-41 at the values of the reference's
+// single-precision literals
+constexpr double kSS18MinScale = (double)0.1f, kSS18Log001 = (double)-4.605170186f,
+                 kSS18LogInvNPairs = (double)-9.210340372f, kSS18NbdShape = (double)0.1f,
+                 kSS18NbdProb = (double)0.757213f, kSS18EbmDfo2 = (double)1.346206f,
+                 kSS18EbmC = (double)1.485656f;
+
+// significance of one alignment, out = [lmbdest, kest, logE, logEpA]: SSEMODEL
+// 0 observed statistics, 1 and 2 the SS18 model and its variant
 __device__ void significance(int ssemodel, int lq, int lt, double qeno, double teno, double searchspace, double score,
                              double psts, double lambda, double K, double* out) {
   const double lref = CTH_UNGAP_LAMBDA, kref = CTH_UNGAP_K, lg = CTH_GAPPED_LAMBDA, kg = CTH_GAPPED_K;
@@ -226,13 +237,22 @@ __device__ void significance(int ssemodel, int lq, int lt, double qeno, double t
   const double l1l2 = (double)(float)((uint32_t)lq * (uint32_t)lt);
   if (lambda <= 0.0 || K <= 0.0) out[3] = log(kref * l1l2) - lref * score;
   else out[3] = log(K * l1l2) - lambda * score;
+  if (ssemodel == 0) {
+    // observed statistics: the gapped reference parameters alone
+    // (CalcSignificanceObsDynPlmProfile -> CalculateEvalueObs,
+    // cuss/CuBatchSS_dynplm.cu:594-660, cuss/CuBatchSS_SS18.cuh:355-381)
+    out[2] = log(kg * searchspace) - lg * score;
+    return;
+  }
+  // PredictEVDParametersSS18 (cuss/CuBatchSS_SS18.cuh:119-233)
   double in[4];
   {
     double e1 = qeno, e2 = teno;
     int n1 = lq, n2 = lt;
     if (e1 < e2) { double t = e1; e1 = e2; e2 = t; int u = n1; n1 = n2; n2 = u; }
     else if (e1 == e2 && n1 < n2) { int u = n1; n1 = n2; n2 = u; }
-    in[0] = 0.05 * e1; in[1] = log((double)n1) / ln1k; in[2] = 0.05 * e2; in[3] = log((double)n2) / ln1k;
+    in[0] = (double)0.05f * e1; in[1] = log((double)n1) / ln1k;
+    in[2] = (double)0.05f * e2; in[3] = log((double)n2) / ln1k;
   }
   double emu = nn_out(in, c_nnw, 30.0);
   {
@@ -241,30 +261,31 @@ __device__ void significance(int ssemodel, int lq, int lt, double qeno, double t
     if (10.0 < lm) lm = 10.0;
     int n1 = lq, n2 = lt;
     if (n1 < n2) { int u = n1; n1 = n2; n2 = u; }
-    in[0] = 0.2 * lm; in[1] = log((double)n1) / ln1k; in[2] = log((double)n2) / ln1k; in[3] = 0.0;
+    in[0] = (double)0.2f * lm; in[1] = log((double)n1) / ln1k; in[2] = log((double)n2) / ln1k; in[3] = 0.0;
   }
   double lmu = nn_out(in, c_nnw + 25, 30.0);
   double lsc = nn_out(in, c_nnw + 50, 5.0);
   const float* adj = ssemodel == 2 ? c_nnw + 84 : c_nnw + 75;
   if (emu < 3.0) emu = 3.0;
   if (lmu < 3.0) lmu = 3.0;
-  if (lsc < 0.1) lsc = 0.1;
+  if (lsc < kSS18MinScale) lsc = kSS18MinScale;
   lsc = adj[6] * (exp(lsc) - 1.0);
-  if (lsc < 0.1) lsc = 0.1;
+  if (lsc < kSS18MinScale) lsc = kSS18MinScale;
   if (5.0 < lsc) lsc = 5.0;
   double scaleE = adj[4] * emu + adj[5];
-  if (scaleE < 0.1) scaleE = 0.1;
+  if (scaleE < kSS18MinScale) scaleE = kSS18MinScale;
   if (5.0 < scaleE) scaleE = 5.0;
   emu = adj[0] * emu + adj[1];
   lmu = adj[2] * lmu + adj[3];
   double alpha = adj[7], ibeta = adj[8];
   if (0.0 < lambda) {
     const double r = 1.0 / lambda;
-    alpha -= alpha * 0.1 * (r < 10.0 ? r : 10.0);
-    ibeta += (1.0 - ibeta) * 0.1 * (r < 10.0 ? r : 10.0);
+    alpha -= alpha * (double)0.1f * (r < 10.0 ? r : 10.0);
+    ibeta += (1.0 - ibeta) * (double)0.1f * (r < 10.0 ? r : 10.0);
   }
   const double mu = alpha * emu + (1.0 - alpha) * lmu;
   const double scale = (1.0 - ibeta) * scaleE + ibeta * lsc;
+  // CalculateEvalueSS18 (cuss/CuBatchSS_SS18.cuh:255-351)
   if (mu <= 0.0 || scale <= 0.0) {
     out[2] = log(kg * searchspace) - lg * score;
     return;
@@ -272,9 +293,9 @@ __device__ void significance(int ssemodel, int lq, int lt, double qeno, double t
   double logK = mu / scale;
   if (K > kref && 0.0 < K && K < 1.0 && 0.0 < kref && kref < 1.0) logK += log(K / kref);
   double logexp = logK - score / scale;
-  if (ssemodel == 2) logexp += -9.210340372;
+  if (ssemodel == 2) logexp += kSS18LogInvNPairs;
   double logp = logexp;
-  if (-4.605170186 <= logexp) {
+  if (kSS18Log001 <= logexp) {
     if (kLogSpMax - 3.0 < logexp) logp = 0.0;
     else {
       const double t = exp(logexp);
@@ -284,10 +305,10 @@ __device__ void significance(int ssemodel, int lq, int lt, double qeno, double t
   }
   const double lp = l1l2 < 625.0 ? 625.0 : l1l2;
   const double pn = rint(100000.0 * psts / pow(lp, 1.5));
-  const double logpp = lbetainc(pn + 1.0, 0.1, 0.757213);
-  logp = lgammaincq(1.346206, (-logp - logpp) / 1.485656);
+  const double logpp = lbetainc(pn + 1.0, kSS18NbdShape, kSS18NbdProb);
+  logp = lgammaincq(kSS18EbmDfo2, (-logp - logpp) / kSS18EbmC);
   logexp = logp;
-  if (-4.605170186 <= logp) {
+  if (kSS18Log001 <= logp) {
     const double p = exp(logp);
     logexp = 77.0;
     if (p < 1.0) logexp = log(-log(1.0 - p));

@@ -36,6 +36,8 @@
 #include "cth_adddist.h"
 #include "cth_pset.h"
 
+void cth_set_error(const char* fmt, ...);
+
 namespace {
 
 // Robinson & Robinson background (libpro/srcpro/SUBSTABLE.cpp:406-412)
@@ -326,6 +328,9 @@ int cth_pset_set_text(cth_pset* ps, int k, const char* name, const char* desc) {
 
 // Allocate an empty profile set with the given lengths.
 cth_pset* cth_pset_alloc(int nprof, const int32_t* lens) {
+  if (nprof < 0 || (nprof > 0 && !lens)) { cth_set_error("pset_alloc: %d profiles", nprof); return nullptr; }
+  for (int p = 0; p < nprof; p++)
+    if (lens[p] < 1) { cth_set_error("pset_alloc: profile %d has length %d", p, (int)lens[p]); return nullptr; }
   cth_pset* ps = (cth_pset*)calloc(1, sizeof(cth_pset));
   if (!ps) return nullptr;
   ps->nprof = nprof;

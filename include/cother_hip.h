@@ -175,6 +175,9 @@ typedef struct cth_search_opts {
                          the matrix cores with FP32 accumulation (no reference counterpart) */
   char* alnbuf;       /* optional: receives the alignment rows of every hit (see cth_hit.alnoff) */
   uint64_t alncap;    /* capacity of alnbuf in bytes */
+  uint64_t dbsize;    /* positions and profiles of the whole database when the resident set is one part of */
+  uint64_t ndb;       /* it (CuBatchProcessing::SetDbDetails, src/libmycu/cupro/CuBatchProcessing.cuh:73);
+                         both 0: the resident set's own.  They set the score threshold and the search space */
 } cth_search_opts;
 
 /* One reported hit; fields follow the reference's per-profile output record

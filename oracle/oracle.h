@@ -96,6 +96,17 @@ double orc_lbetainc(double a, double b, double x);
 double orc_lgammaincq(double a, double x);
 void orc_significance(int ssemodel, int lq, int lt, double qeno, double teno, double searchspace, double score,
                       double psts, double lambda, double K, double out[4]);
+/* the steps of the SS18 E-value, for the tests */
+void orc_ss18_weights(float out[93]);
+double orc_ss18_nn(int which, const double in[4]);
+void orc_ss18_input_e(int lq, int lt, double qeno, double teno, double in[4]);
+void orc_ss18_input_l(int lq, int lt, double lambda, double in[4]);
+void orc_ss18_predict(int ssemodel, int lq, int lt, double qeno, double teno, double lambda, double* mu,
+                      double* scale);
+double orc_ss18_score_logp(int ssemodel, double score, double K, double mu, double scale);
+double orc_ss18_psts_logp(int lq, int lt, double psts);
+double orc_ss18_combine(double logp_score, double logp_psts);
+double orc_ss18_logevalue(double logp, double searchspace, int lq, int lt);
 
 /* MAP alignment stage (orc_map.c) */
 typedef struct orc_mapres {

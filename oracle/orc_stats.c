@@ -321,9 +321,35 @@ double orc_lgammaincq(double a, double x) {
   return -x + a * log(x) - gln + log(h);
 }
 
-/* ---- SS18 significance ------------------------------------------------------ */
+/* ---- SS18 significance ------------------------------------------------------
+ * Constants of cuss/CuBatchSS_SS18.cuh:17-41 and cuss/CuBatchSS_com.h:39; the
+ * reference's values are single-precision literals, used here at their float
+ * values. */
+#define SS18_MINMU 3.0
+#define SS18_MINSCALE ((double)0.1f)
+#define SS18_MAXSCALE 5.0
+#define SS18_LOG001 ((double)-4.605170186f)
+#define SS18_LOGINVNPAIRS ((double)-9.210340372f)
+#define SS18_PSTS_FACTOR 100000.0
+#define SS18_PSTS_EXP 1.5
+#define SS18_NBD_SHAPE ((double)0.1f)
+#define SS18_NBD_PROB ((double)0.757213f)
+#define SS18_EBM_DFO2 ((double)1.346206f)
+#define SS18_EBM_C ((double)1.485656f)
+#define SS18_PSTS_MIN_LENPRODUCT 625.0
+#define SS18_LN1K 6.90775527898213681510242167860
+#define REF_LAMBDA ((double)1.116276f)
+#define REF_K ((double)0.115812f)
+#define GAP_LAMBDA ((double)0.561132f)
+#define GAP_K ((double)0.001488f)
+
+void orc_ss18_weights(float out[93]) { memcpy(out, NNW, sizeof NNW); }
+
 static double sig2(double v) { return 2.0 / (1.0 + exp(-v)) - 1.0; }
 
+/* one network: hidden unit h has bias w[5h] and input weights w[5h+1..5h+4];
+   the output has bias w[20] and unit weights w[21..24]; activation
+   2/(1+e^-x) - 1 at both layers (cuss/CuBatchSS_NN.cuh:37-148) */
 static double nn_out(const double in[4], const float* w, double factor) {
   double acc = w[20];
   for (int h = 0; h < 4; h++) {
@@ -334,69 +360,78 @@ static double nn_out(const double in[4], const float* w, double factor) {
   return sig2(acc) * factor;
 }
 
-/* out: [lmbdest, kest, logE, logEpA] */
-void orc_significance(int ssemodel, int lq, int lt, double qeno, double teno, double searchspace, double score,
-                      double psts, double lambda, double K, double out[4]) {
-  const double lref = 1.116276f, kref = 0.115812f, lg = 0.561132f, kg = 0.001488f;
-  const double ln1k = 6.90775527898213681510242167860;
-  out[0] = 0.0; out[1] = 0.0;
-  if (0.0 < lambda) out[0] = lambda * (lg / lref);
-  if (0.0 < K) out[1] = K * (kg / kref);
-  const double l1l2 = (double)(float)((uint32_t)lq * (uint32_t)lt);
-  if (lambda <= 0.0 || K <= 0.0) out[3] = log(kref * l1l2) - lref * score;
-  else out[3] = log(K * l1l2) - lambda * score;
-  /* PredictEVDParametersSS18 */
+/* which 0: mu from profile attributes (E), 1: mu and 2: scale from the
+   compositional similarity (L) */
+double orc_ss18_nn(int which, const double in[4]) {
+  return nn_out(in, NNW + 25 * which, which == 2 ? 5.0 : 30.0);
+}
+
+/* FormatInputConfigurationE (cuss/CuBatchSS_NN.cuh:150-176) */
+void orc_ss18_input_e(int lq, int lt, double qeno, double teno, double in[4]) {
+  double e1 = qeno, e2 = teno;
+  int n1 = lq, n2 = lt;
+  if (e1 < e2) { double t = e1; e1 = e2; e2 = t; int u = n1; n1 = n2; n2 = u; }
+  else if (e1 == e2 && n1 < n2) { int u = n1; n1 = n2; n2 = u; }
+  in[0] = (double)0.05f * e1; in[1] = log((double)n1) / SS18_LN1K;
+  in[2] = (double)0.05f * e2; in[3] = log((double)n2) / SS18_LN1K;
+}
+
+/* FormatInputConfigurationL (cuss/CuBatchSS_NN.cuh:178-202) */
+void orc_ss18_input_l(int lq, int lt, double lambda, double in[4]) {
+  double lm = rint(lambda * 10.0) / 10.0;
+  if (lm <= 0.0) lm = 0.1;
+  if (10.0 < lm) lm = 10.0;
+  int n1 = lq, n2 = lt;
+  if (n1 < n2) { int u = n1; n1 = n2; n2 = u; }
+  in[0] = (double)0.2f * lm; in[1] = log((double)n1) / SS18_LN1K; in[2] = log((double)n2) / SS18_LN1K; in[3] = 0.0;
+}
+
+/* PredictEVDParametersSS18 (cuss/CuBatchSS_SS18.cuh:119-233): the EVD's mu
+   and scale from the three networks and the adjustment parameters of the
+   model's variant (9 values each, CuBatchProcessing_com.cuh:61-72; variant 1
+   from weight 75, variant 2 from weight 84) */
+void orc_ss18_predict(int ssemodel, int lq, int lt, double qeno, double teno, double lambda, double* mu,
+                      double* scale) {
   double in[4];
-  {
-    double e1 = qeno, e2 = teno;
-    int n1 = lq, n2 = lt;
-    if (e1 < e2) { double t = e1; e1 = e2; e2 = t; int u = n1; n1 = n2; n2 = u; }
-    else if (e1 == e2 && n1 < n2) { int u = n1; n1 = n2; n2 = u; }
-    in[0] = 0.05 * e1; in[1] = log((double)n1) / ln1k; in[2] = 0.05 * e2; in[3] = log((double)n2) / ln1k;
-  }
-  double emu = nn_out(in, NNW, 30.0);
-  {
-    double lm = rint(lambda * 10.0) / 10.0;
-    if (lm <= 0.0) lm = 0.1;
-    if (10.0 < lm) lm = 10.0;
-    int n1 = lq, n2 = lt;
-    if (n1 < n2) { int u = n1; n1 = n2; n2 = u; }
-    in[0] = 0.2 * lm; in[1] = log((double)n1) / ln1k; in[2] = log((double)n2) / ln1k; in[3] = 0.0;
-  }
-  double lmu = nn_out(in, NNW + 25, 30.0);
-  double lsc = nn_out(in, NNW + 50, 5.0);
-  const float* adj = NNW + 75;
-  if (ssemodel == 2) adj = NNW + 84;
-  if (emu < 3.0) emu = 3.0;
-  if (lmu < 3.0) lmu = 3.0;
-  if (lsc < 0.1) lsc = 0.1;
+  orc_ss18_input_e(lq, lt, qeno, teno, in);
+  double emu = orc_ss18_nn(0, in);
+  orc_ss18_input_l(lq, lt, lambda, in);
+  double lmu = orc_ss18_nn(1, in);
+  double lsc = orc_ss18_nn(2, in);
+  const float* adj = ssemodel == 2 ? NNW + 84 : NNW + 75;
+  if (emu < SS18_MINMU) emu = SS18_MINMU;
+  if (lmu < SS18_MINMU) lmu = SS18_MINMU;
+  if (lsc < SS18_MINSCALE) lsc = SS18_MINSCALE;
   lsc = adj[6] * (exp(lsc) - 1.0);
-  if (lsc < 0.1) lsc = 0.1;
-  if (5.0 < lsc) lsc = 5.0;
+  if (lsc < SS18_MINSCALE) lsc = SS18_MINSCALE;
+  if (SS18_MAXSCALE < lsc) lsc = SS18_MAXSCALE;
   double scaleE = adj[4] * emu + adj[5];
-  if (scaleE < 0.1) scaleE = 0.1;
-  if (5.0 < scaleE) scaleE = 5.0;
+  if (scaleE < SS18_MINSCALE) scaleE = SS18_MINSCALE;
+  if (SS18_MAXSCALE < scaleE) scaleE = SS18_MAXSCALE;
   emu = adj[0] * emu + adj[1];
   lmu = adj[2] * lmu + adj[3];
   double alpha = adj[7], ibeta = adj[8];
   if (0.0 < lambda) {
     const double r = 1.0 / lambda;
-    alpha -= alpha * 0.1 * (r < 10.0 ? r : 10.0);
-    ibeta += (1.0 - ibeta) * 0.1 * (r < 10.0 ? r : 10.0);
+    alpha -= alpha * (double)0.1f * (r < 10.0 ? r : 10.0);
+    ibeta += (1.0 - ibeta) * (double)0.1f * (r < 10.0 ? r : 10.0);
   }
-  const double mu = alpha * emu + (1.0 - alpha) * lmu;
-  const double scale = (1.0 - ibeta) * scaleE + ibeta * lsc;
-  /* CalculateEvalueSS18 */
-  if (mu <= 0.0 || scale <= 0.0) {
-    out[2] = log(kg * searchspace) - lg * score;
-    return;
-  }
+  *mu = alpha * emu + (1.0 - alpha) * lmu;
+  *scale = (1.0 - ibeta) * scaleE + ibeta * lsc;
+}
+
+/* CalculateEvalueSS18 (cuss/CuBatchSS_SS18.cuh:255-351), step 1: log p-value
+   of the score under the EVD (Gumbel) with location mu and scale,
+   P(S >= s) = 1 - exp(-exp(-(s - mu')/scale)), mu' = mu + scale log(K/Kref)
+   when K > Kref, for variant 2 with the expected count divided by 10^4; an
+   expected count below 0.01 is taken as the p-value itself */
+double orc_ss18_score_logp(int ssemodel, double score, double K, double mu, double scale) {
   double logK = mu / scale;
-  if (K > kref && 0.0 < K && K < 1.0 && 0.0 < kref && kref < 1.0) logK += log(K / kref);
+  if (K > REF_K && 0.0 < K && K < 1.0 && 0.0 < REF_K && REF_K < 1.0) logK += log(K / REF_K);
   double logexp = logK - score / scale;
-  if (ssemodel == 2) logexp += -9.210340372;
+  if (ssemodel == 2) logexp += SS18_LOGINVNPAIRS;
   double logp = logexp;
-  if (-4.605170186 <= logexp) {
+  if (SS18_LOG001 <= logexp) {
     if (LOG_SP_MAX - 3.0 < logexp) logp = 0.0;
     else {
       const double t = exp(logexp);
@@ -404,16 +439,62 @@ void orc_significance(int ssemodel, int lq, int lt, double qeno, double teno, do
       else logp = log(1.0 - exp(-t));
     }
   }
-  const double lp = l1l2 < 625.0 ? 625.0 : l1l2;
-  const double pn = rint(100000.0 * psts / pow(lp, 1.5));
-  const double logpp = orc_lbetainc(pn + 1.0, 0.1, 0.757213);
-  logp = orc_lgammaincq(1.346206, (-logp - logpp) / 1.485656);
-  logexp = logp;
-  if (-4.605170186 <= logp) {
+  return logp;
+}
+
+/* step 2: log p-value of the positive-score count, the negative binomial
+   tail I_x(n + 1, 0.1) at x = 0.757213 of the normalised count
+   n = rint(10^5 psts / max(Lq Lt, 625)^1.5) */
+double orc_ss18_psts_logp(int lq, int lt, double psts) {
+  const double l1l2 = (double)(float)((uint32_t)lq * (uint32_t)lt);
+  const double lp = l1l2 < SS18_PSTS_MIN_LENPRODUCT ? SS18_PSTS_MIN_LENPRODUCT : l1l2;
+  const double pn = rint(SS18_PSTS_FACTOR * psts / pow(lp, SS18_PSTS_EXP));
+  return orc_lbetainc(pn + 1.0, SS18_NBD_SHAPE, SS18_NBD_PROB);
+}
+
+/* step 3: the two combined by Brown's method: chi-square-like statistic
+   (-log p1 - log p2) / c with 2 k degrees of freedom, p = Q(k, .) */
+double orc_ss18_combine(double logp_score, double logp_psts) {
+  return orc_lgammaincq(SS18_EBM_DFO2, (-logp_score - logp_psts) / SS18_EBM_C);
+}
+
+/* step 4: p-value to log E-value (E = -log(1 - p); p itself below 0.01; 77
+   for p = 1), scaled from the pair's Lq Lt to the search space */
+double orc_ss18_logevalue(double logp, double searchspace, int lq, int lt) {
+  const double l1l2 = (double)(float)((uint32_t)lq * (uint32_t)lt);
+  double logexp = logp;
+  if (SS18_LOG001 <= logp) {
     const double p = exp(logp);
     logexp = 77.0;
     if (p < 1.0) logexp = log(-log(1.0 - p));
   }
-  logexp += log(searchspace / l1l2);
-  out[2] = logexp;
+  return logexp + log(searchspace / l1l2);
+}
+
+/* out: [lmbdest, kest, logE, logEpA]
+   EstimateGappedLambdaK and GetPairEvalue (cuss/CuBatchSS_SS18.cuh:384-453),
+   then the log E-value of the model: SSEMODEL 0 CalcSignificanceObsDynPlmProfile
+   (cuss/CuBatchSS_dynplm.cu:594-660, CalculateEvalueObs), 1 and 2
+   CalcSignificanceSS18DynPlmProfile (cuss/CuBatchSS_dynplm.cu:447-580) */
+void orc_significance(int ssemodel, int lq, int lt, double qeno, double teno, double searchspace, double score,
+                      double psts, double lambda, double K, double out[4]) {
+  out[0] = 0.0; out[1] = 0.0;
+  if (0.0 < lambda) out[0] = lambda * (GAP_LAMBDA / REF_LAMBDA);
+  if (0.0 < K) out[1] = K * (GAP_K / REF_K);
+  const double l1l2 = (double)(float)((uint32_t)lq * (uint32_t)lt);
+  if (lambda <= 0.0 || K <= 0.0) out[3] = log(REF_K * l1l2) - REF_LAMBDA * score;
+  else out[3] = log(K * l1l2) - lambda * score;
+  if (ssemodel == 0) {
+    out[2] = log(GAP_K * searchspace) - GAP_LAMBDA * score;
+    return;
+  }
+  double mu, scale;
+  orc_ss18_predict(ssemodel, lq, lt, qeno, teno, lambda, &mu, &scale);
+  if (mu <= 0.0 || scale <= 0.0) {
+    out[2] = log(GAP_K * searchspace) - GAP_LAMBDA * score;
+    return;
+  }
+  const double lps = orc_ss18_score_logp(ssemodel, score, K, mu, scale);
+  const double lpp = orc_ss18_psts_logp(lq, lt, psts);
+  out[2] = orc_ss18_logevalue(orc_ss18_combine(lps, lpp), searchspace, lq, lt);
 }

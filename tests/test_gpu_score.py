@@ -27,6 +27,24 @@ def test_score_matrix_bit_exact(ctx, oracle, sets):
         assert (md.view(np.int32) == omd.view(np.int32)).all()
 
 
+def test_score_matrix_informative_context_vectors(ctx, oracle):
+    """Random context vectors, norms and priors (the CVS term over all three
+    branches of its map): bit-exact."""
+    from tests._cv import set_informative_cv
+    q = set_informative_cv(cother_amd.synth(2, 0, lmin=50, lmax=120, dist_mode=0, sort_desc=0, seed=111), 112)
+    db = set_informative_cv(cother_amd.synth(30, 0, lmin=20, lmax=300, dist_mode=1, sort_desc=1, seed=113), 114)
+    ctx.upload_queries(q)
+    ctx.upload_db(db)
+    for qi in range(q.nprof):
+        sc, md = ctx.score_block(qi, 0, db.nprof)
+        osc, omd = oracle.score_matrix(q, qi, db, 0, db.nprof)
+        bad = np.argwhere(sc.view(np.int32) != osc.view(np.int32))
+        assert bad.size == 0, f"query {qi}: {len(bad)} cells differ, first {bad[:5].tolist()}"
+        assert (md.view(np.int32) == omd.view(np.int32)).all()
+    # the modular scores (SS + CVS terms) are not constant across cells
+    assert np.unique(md).size > 1000
+
+
 def test_score_matrix_subrange(ctx, oracle, sets):
     q, db = sets
     ctx.upload_queries(q)

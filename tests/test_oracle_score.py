@@ -259,3 +259,60 @@ def test_score_matrix_composition(oracle, small_sets):
             want = np.float32(want + d)
         assert sc[r, c] == want
         assert abs(md[r, c] - ((s2 - 0.075) + ((d - 0.04) if d != 0 else 0))) < 1e-6
+
+
+def _cvs_float64(T, qcv, qn2, qpr, dcv, dn2, dpr):
+    """The CVS term in float64 from the formula of CalcCVS2ScoreSMEMUnroll2x
+    (cusco/CuBatchScoreMatrix_cvs2s.cuh:80-113; GetScoreE1S1Step2Boundary,
+    cupro/SerializedCVS2Scores.cuh:377-420): the log-odds of the two vectors
+    under the normal-Wishart model, det = (|q|^2 + 1 + 1/k0)(|t|^2 + 1 + 1/k0)
+    - (q.t + 1/k0)^2, mapped through the smoothed cvs2s table; returns the
+    mapped score x 0.15 and the log-odds before the map."""
+    k = float(T.cvkappa0inv)
+    dot = float(np.dot(qcv.astype(np.float64), dcv.astype(np.float64))) + k
+    det = (float(qn2) + 1.0 + k) * (float(dn2) + 1.0 + k) - dot * dot
+    if det <= 0.0:  # only for inconsistent inputs (a vector that does not match its norm)
+        return math.nan, math.nan
+    x = float(T.cvcterm) + float(T.cvpowernu0) * math.log(det) - float(qpr) - float(dpr)
+    keys, vals, card = np.ctypeslib.as_array(T.cvkeys), np.ctypeslib.as_array(T.cvvals), T.cvcard
+    if x <= keys[0]:
+        v = vals[0]
+    elif keys[card - 1] <= x:
+        v = vals[card - 1]
+    else:
+        v = vals[T.cvshift + int(np.rint(2.0 * x))]
+    return float(v) * 0.15, x
+
+
+def test_cvs_term_informative_vectors(oracle):
+    """orc_cell_cvs on random context vectors (entries, squared norms and log
+    priors as text profiles carry them) against the float64 formula, over all
+    three branches of the map; a transposed vector stride or a dropped prior
+    gives other values."""
+    from tests._cv import set_informative_cv
+    q = set_informative_cv(cother_amd.synth(3, 0, lmin=30, lmax=60, dist_mode=2, sort_desc=0, seed=71), 72)
+    db = set_informative_cv(cother_amd.synth(8, 0, lmin=30, lmax=90, dist_mode=2, sort_desc=0, seed=73), 74)
+    T = oracle.tables
+    vp = ctypes.c_void_p
+    qcv, qn2, qpr = q.field("cv"), q.field("cvnorm2"), q.field("cvprior")
+    dcv, dn2, dpr = db.field("cv"), db.field("cvnorm2"), db.field("cvprior")
+    keys = np.ctypeslib.as_array(T.cvkeys)
+    rng = np.random.default_rng(75)
+    branch = [0, 0, 0]
+    checked = stride_bad = prior_bad = 0
+    for _ in range(3000):
+        qi, tj = int(rng.integers(0, q.npos)), int(rng.integers(0, db.npos))
+        want, x = _cvs_float64(T, qcv[:, qi], qn2[qi], qpr[qi], dcv[:, tj], dn2[tj], dpr[tj])
+        if abs(2.0 * x - np.floor(2.0 * x) - 0.5) < 1e-3:
+            continue  # single precision may round to the neighbouring entry
+        got = oracle.L.orc_cell_cvs(ctypes.byref(T), ctypes.cast(q.ptr, vp), qi, ctypes.cast(db.ptr, vp), tj)
+        assert abs(got - want) < 1e-6, (qi, tj, x)
+        checked += 1
+        branch[0 if x <= keys[0] else (2 if x >= keys[T.cvcard - 1] else 1)] += 1
+        # planted mistakes, in the float64 formula
+        tv = qcv.reshape(-1)[qi * 19:(qi + 1) * 19] if (qi + 1) * 19 <= qcv.size else qcv[:, qi]
+        stride_bad += not abs(_cvs_float64(T, tv, qn2[qi], qpr[qi], dcv[:, tj], dn2[tj], dpr[tj])[0] - want) <= 1e-4
+        prior_bad += not abs(_cvs_float64(T, qcv[:, qi], qn2[qi], 0.0, dcv[:, tj], dn2[tj], dpr[tj])[0] - want) <= 1e-4
+    assert checked > 2500
+    assert min(branch) > 50, branch
+    assert stride_bad > checked // 2 and prior_bad > checked // 2

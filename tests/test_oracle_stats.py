@@ -10,6 +10,7 @@
 - the search-space length adjustment against its fixed-point equation.
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -138,3 +139,38 @@ def test_length_adjustment_fixed_point(oracle):
         assert dl <= rhs + 1e-3 and rhs - dl < 1.0 + 1e-3
         # the search space is that of the last length iterate, between dl and dl+1
         assert (m - dl - 1) * (n - N * (dl + 1)) * (1 - 1e-6) <= ss <= (m - dl) * (n - N * dl) * (1 + 1e-6)
+
+
+LENADJ = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref", "lenadj_ref")
+
+
+@pytest.mark.skipif(not os.path.exists(LENADJ), reason="reference length adjustment not built (oracle/Makefile.ref)")
+def test_length_adjustment_equals_compiled_reference(oracle):
+    """The search space and length adjustment of the oracle and of the
+    product's host code (cth_search_space) equal those of the reference's own
+    AbstractScoreMatrix::ComputeLengthAdjustment (AbstractScoreMatrix.cpp:
+    303-395, built from source into oracle/_ref/lenadj_ref), on query lengths,
+    database sizes and counts across the configurations and beyond, including
+    the cases with no admissible length (c < 0) and those that need the
+    bisection."""
+    import subprocess
+    import cother_amd
+    rng = np.random.default_rng(77)
+    cases = [(150, 447922, 2000), (250, 14074887, 60000), (1000, 14074887, 60000), (2000, 9950000, 5000),
+             (40, 10000, 50), (1, 1, 1), (5000, 10 ** 9, 10 ** 6)]
+    for _ in range(3000):
+        m = int(rng.integers(1, 5001))
+        n = int(10 ** rng.uniform(2, 9.5))
+        N = int(max(1, min(n, 10 ** rng.uniform(0, 6.5))))
+        cases.append((m, n, N))
+    inp = "".join(f"{m} {n} {N}\n" for m, n, N in cases)
+    r = subprocess.run([LENADJ, "0.561132", "0.001488", "0.5", "-1"], input=inp, capture_output=True, text=True,
+                       check=True)
+    ref = [tuple(int(x) for x in l.split()) for l in r.stdout.splitlines()]
+    assert len(ref) == len(cases)
+    adjusted = 0
+    for (m, n, N), (rss, rdl) in zip(cases, ref):
+        assert cother_amd.search_space(m, n, N) == (rss, rdl), (m, n, N)
+        assert oracle.length_adjustment(m, n, N) == (rss, rdl), (m, n, N)
+        adjusted += rdl > 0
+    assert adjusted > 500  # most cases take the iteration, not the early return

@@ -44,8 +44,12 @@ def _same(a, b, skip=()):
 
 @pytest.fixture(scope="module")
 def sets():
+    from tests._cv import set_informative_cv
+    # the third set carries context vectors ("CV:" lines) at about half its positions
     return [cother_amd.synth(6, 0, lmin=5, lmax=120, dist_mode=1, sort_desc=0, seed=201),
-            cother_amd.synth(4, 0, lmin=40, lmax=90, dist_mode=0, sort_desc=0, seed=202)]
+            cother_amd.synth(4, 0, lmin=40, lmax=90, dist_mode=0, sort_desc=0, seed=202),
+            set_informative_cv(cother_amd.synth(3, 0, lmin=20, lmax=70, dist_mode=1, sort_desc=0, seed=203), 204,
+                               frac=0.5)]
 
 
 def _ss_written(v, scale=10000):
