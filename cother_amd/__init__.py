@@ -119,6 +119,7 @@ def lib():
     L.cth_bench_copy.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
     L.cth_search.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(SearchOpts),
                              ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(SearchStats)]
+    L.cth_search_noalloc.argtypes = L.cth_search.argtypes
     L.cth_pset_name.restype = ctypes.c_char_p
     L.cth_pset_name.argtypes = [P, ctypes.c_int]
     L.cth_pset_desc.restype = ctypes.c_char_p
@@ -479,11 +480,12 @@ class Context:
         return res
 
     def search(self, q0=0, q1=None, eval=10.0, ssemodel=2, do_map=True, timing=False, mem_budget=0,
-               maxhits=None, minpp=MINPP, alignments=False, contraction=0, dbsize=0, ndb=0):
+               maxhits=None, minpp=MINPP, alignments=False, contraction=0, dbsize=0, ndb=0, noalloc=False):
         """Search queries [q0,q1) against the resident database.
 
         dbsize, ndb: positions and profiles of the whole database when the resident set is one
-        part of it (0, 0: the resident set's own).
+        part of it (0, 0: the resident set's own).  noalloc: run cth_search_noalloc, whose
+        buffers come from the library's fixed per-device scratch array (mem_budget unused).
 
         Returns (hits HIT array grouped by query, per-query hit counts, SearchStats), and with
         alignments=True also a list of five-row alignments (query, middle, target, query SS,
@@ -502,8 +504,9 @@ class Context:
         opts = SearchOpts(eval, ssemodel, int(do_map), int(timing), mem_budget, minpp, contraction,
                           ctypes.cast(abuf, ctypes.c_void_p) if abuf is not None else None,
                           len(abuf) if abuf is not None else 0, dbsize, ndb)
-        _check(lib().cth_search(self.h, q0, q1, ctypes.byref(opts), hits.ctypes.data, maxhits, nh.ctypes.data,
-                                ctypes.byref(st)), "cth_search")
+        fn = lib().cth_search_noalloc if noalloc else lib().cth_search
+        _check(fn(self.h, q0, q1, ctypes.byref(opts), hits.ctypes.data, maxhits, nh.ctypes.data, ctypes.byref(st)),
+               "cth_search_noalloc" if noalloc else "cth_search")
         hits = hits[:int(nh.sum())]
         if not alignments:
             return hits, nh, st

@@ -69,6 +69,8 @@ extern "C" int cth_ctx_create(int device, const char* envdir, cth_ctx** out) {
       hipMemcpy(c->dtab, &c->htab, sizeof(cth_tables), hipMemcpyHostToDevice) != hipSuccess) {
     cth_set_error("cannot upload score tables"); cth_ctx_destroy(c); return -1;
   }
+  // the device address of cth_search_noalloc's scratch array (once per device)
+  if (cth_scratch_init(device)) { cth_ctx_destroy(c); return -1; }
   *out = c;
   return 0;
 }

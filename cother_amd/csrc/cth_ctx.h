@@ -37,4 +37,6 @@ struct cth_ctx {
 extern "C" void cth_ctx_trim(cth_ctx* c, size_t keep);
 
 int cth_ctx_reserve(cth_ctx* c, void** p, size_t* cap, size_t need);
+// set up the device address of cth_search_noalloc's scratch array (once per device, device current)
+extern "C" int cth_scratch_init(int device);
 extern "C" void cth_ctx_destroy(cth_ctx* c);

@@ -22,6 +22,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "cth_ctx.h"
@@ -103,7 +104,129 @@ double now_ms() {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// Working memory of cth_search_noalloc: one fixed array that every device
+// running the library holds, at its own address, until the process ends.
+// In host code `scratch` names a host-side stand-in, not device memory: the
+// device address comes from hipGetSymbolAddress (scratch_init) and only that
+// address is passed to kernels.  No buffer carved from it needs to start at
+// zero: every kernel writes the cells, records and lists it reads later in
+// the same call.
+static __device__ __align__(256) uint8_t scratch[CTH_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch_addr[kMaxDevices];  // device address of scratch per device (null until set up)
+
+uint8_t* scratch_of(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return (device >= 0 && device < kMaxDevices) ? g_scratch_addr[device] : nullptr;
+}
+
+// Device buffers of one database chunk of the search.  cth_search grows the
+// context's own buffers (allocating when a chunk needs more than they hold);
+// cth_search_noalloc lays them out in the fixed scratch region: scores,
+// modular scores, traceback bytes, phase-1 records and MAP posteriors at the
+// front, then the work area that each later stage reuses for its lists and
+// records (a stage's work contents are dead once the next stage takes it).
+struct chunk_bufs {
+  cth_ctx* c;
+  uint8_t* fixed;       // the fixed region, or null to grow the context's buffers
+  size_t fixed_cap;
+  const char* who;      // entry-point name for error messages
+  size_t work_off = 0;  // fixed mode: start of the work area
+  float* scores = nullptr;
+  float* mods = nullptr;
+  uint8_t* btck = nullptr;
+  cth_dpres* dpout = nullptr;
+  float* mapbuf = nullptr;
+
+  chunk_bufs(cth_ctx* ctx, uint8_t* base, size_t cap, const char* name)
+      : c(ctx), fixed(base), fixed_cap(cap), who(name) {}
+
+  static size_t up(size_t n) { return (n + 255) & ~(size_t)255; }
+  // fixed mode: the share of the region the chunk buffers may take (the rest is the work area)
+  size_t chunk_share() const { return fixed_cap / 4 * 3; }
+  // fixed mode: the most columns a chunk may span so that its layout fits the
+  // share: with ldsc <= cols + 63 and at most one profile per column, the
+  // layout needs at most lq*(cols+63)*bpc + 24*cols + 5*256 bytes
+  int64_t max_chunk_cols(int lq, size_t bpc) const {
+    const int64_t fixed_part = 5 * 256 + 63 * (int64_t)lq * (int64_t)bpc;
+    const int64_t room = (int64_t)chunk_share() - fixed_part;
+    return room > 0 ? room / ((int64_t)lq * (int64_t)bpc + (int64_t)sizeof(cth_dpres)) : 0;
+  }
+
+  int chunk(int qi, int lq, int ltchunk, int64_t ldsc, int np, bool map) {
+    const size_t cells = (size_t)lq * (size_t)ldsc;
+    if (!fixed) {
+      if (cth_ctx_reserve(c, (void**)&c->scores, &c->cap_scores, 4 * cells)) return -1;
+      if (cth_ctx_reserve(c, (void**)&c->mods, &c->cap_mods, 4 * cells)) return -1;
+      if (cth_ctx_reserve(c, (void**)&c->btck, &c->cap_btck, cells)) return -1;
+      if (cth_ctx_reserve(c, (void**)&c->dpout, &c->cap_dpout, sizeof(cth_dpres) * (size_t)np)) return -1;
+      scores = c->scores; mods = c->mods; btck = c->btck; dpout = (cth_dpres*)c->dpout;
+      mapbuf = nullptr;
+      return 0;
+    }
+    const size_t need = up(4 * cells) * (map ? 3 : 2) + up(cells) + up(sizeof(cth_dpres) * (size_t)np);
+    if (need > chunk_share()) {
+      cth_set_error("%s: query %d (%d positions) against a database chunk of %lld columns (longest profile %d) "
+                    "needs %zu bytes of buffers; the scratch allows %zu",
+                    who, qi, lq, (long long)ldsc, ltchunk, need, chunk_share());
+      return -1;
+    }
+    size_t o = 0;
+    auto take = [&](size_t n) { uint8_t* p = fixed + o; o += up(n); return p; };
+    scores = (float*)take(4 * cells);
+    mods = (float*)take(4 * cells);
+    btck = take(cells);
+    dpout = (cth_dpres*)take(sizeof(cth_dpres) * (size_t)np);
+    mapbuf = map ? (float*)take(4 * cells) : nullptr;
+    work_off = o;
+    return 0;
+  }
+
+  // the MAP posterior buffer of the current chunk
+  float* map_buffer(int lq, int64_t ldsc) {
+    if (fixed) {
+      if (!mapbuf) cth_set_error("%s: no MAP buffer in this chunk's layout", who);
+      return mapbuf;
+    }
+    if (cth_ctx_reserve(c, (void**)&c->mapbuf, &c->cap_mapbuf, sizeof(float) * (size_t)lq * (size_t)ldsc))
+      return nullptr;
+    return mapbuf = c->mapbuf;
+  }
+
+  // the work area of a stage (null with the error set when it does not fit)
+  void* work(size_t need) {
+    if (!fixed) {
+      if (cth_ctx_reserve(c, &c->work, &c->cap_work, need)) return nullptr;
+      return c->work;
+    }
+    if (need > fixed_cap - work_off) {
+      cth_set_error("%s: a stage needs %zu bytes of lists and records; %zu are left after the chunk buffers", who,
+                    need, fixed_cap - work_off);
+      return nullptr;
+    }
+    return fixed + work_off;
+  }
+};
+
 }  // namespace
+
+// Set up this device's scratch address (once per device, with the device
+// current); contexts call it at creation.
+extern "C" int cth_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) { cth_set_error("scratch: device %d out of range", device); return -1; }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_addr[device]) return 0;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    cth_set_error("scratch: hipGetSymbolAddress on device %d: %s", device, hipGetErrorString(e));
+    return -1;
+  }
+  g_scratch_addr[device] = (uint8_t*)p;
+  return 0;
+}
 
 extern "C" void cth_search_space(uint64_t qlen, uint64_t dbsize, uint64_t ndb, uint64_t* sspace, uint64_t* deltalen) {
   uint64_t dl = 0;
@@ -130,11 +253,15 @@ extern "C" int cth_search_opts_check(const cth_search_opts* opt, const char* who
   return 0;
 }
 
-extern "C" int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt, cth_hit* hits, int maxhits,
-                          int* nhits, cth_search_stats* stats) {
-  if (!c->has_q || !c->has_db) { cth_set_error("search: queries or database not uploaded"); return -1; }
-  if (q0 < 0 || q1 > c->q.d.nprof || q0 >= q1) { cth_set_error("search: bad query range [%d,%d)", q0, q1); return -1; }
-  if (cth_search_opts_check(opt, "search")) return -1;
+namespace {
+
+// The whole per-query pipeline; its per-chunk device buffers come from bufs.
+int search_run(cth_ctx* c, int q0, int q1, const cth_search_opts* opt, cth_hit* hits, int maxhits, int* nhits,
+               cth_search_stats* stats, chunk_bufs& bufs) {
+  const char* who = bufs.who;
+  if (!c->has_q || !c->has_db) { cth_set_error("%s: queries or database not uploaded", who); return -1; }
+  if (q0 < 0 || q1 > c->q.d.nprof || q0 >= q1) { cth_set_error("%s: bad query range [%d,%d)", who, q0, q1); return -1; }
+  if (cth_search_opts_check(opt, who)) return -1;
   CTH_HIP_CHECK(hipSetDevice(c->device));
   const cth_devset& db = c->db;
   const int nprof = db.d.nprof;
@@ -165,7 +292,9 @@ extern "C" int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt
     const size_t bytes_per_cell = opt->do_map ? 13 : 9;
     constexpr int64_t kMaxChunkCols = 6 << 20;
     const int64_t chunkcols =
-        std::max<int64_t>(4096, std::min<int64_t>(kMaxChunkCols, (int64_t)(budget / ((size_t)lq * bytes_per_cell))));
+        bufs.fixed ? std::min<int64_t>(kMaxChunkCols, bufs.max_chunk_cols(lq, bytes_per_cell))
+                   : std::max<int64_t>(4096, std::min<int64_t>(kMaxChunkCols,
+                                                               (int64_t)(budget / ((size_t)lq * bytes_per_cell))));
     const int qh0 = nh;
     for (int p0 = 0; p0 < nprof;) {
       int p1 = p0;
@@ -173,14 +302,17 @@ extern "C" int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt
       while (p1 < nprof && (db.h_dist[p1] + db.h_len[p1] - col0 <= chunkcols || p1 == p0)) p1++;
       const int64_t ncols = db.h_dist[p1 - 1] + db.h_len[p1 - 1] - col0;
       const int64_t ldsc = (ncols + 63) / 64 * 64;
-      const size_t need = sizeof(float) * (size_t)lq * ldsc;
-      if (cth_ctx_reserve(c, (void**)&c->scores, &c->cap_scores, need)) return -1;
-      if (cth_ctx_reserve(c, (void**)&c->mods, &c->cap_mods, need)) return -1;
-      if (cth_ctx_reserve(c, (void**)&c->btck, &c->cap_btck, (size_t)lq * ldsc)) return -1;
-      if (cth_ctx_reserve(c, (void**)&c->dpout, &c->cap_dpout, sizeof(cth_dpres) * (size_t)(p1 - p0))) return -1;
+      int ltchunk = 0;
+      for (int p = p0; p < p1; p++) ltchunk = std::max(ltchunk, db.h_len[p]);
+      if (bufs.chunk(qi, lq, ltchunk, ldsc, p1 - p0, opt->do_map != 0)) return -1;
+      float* const scores = bufs.scores;
+      float* const mods = bufs.mods;
+      uint8_t* const btck = bufs.btck;
+      cth_dpres* const dpout = bufs.dpout;
       const double t0 = now_ms();
       if (timing) CTH_HIP_CHECK(hipEventRecord(c->ev[0], c->st));
-      if (cth_launch_score(&c->q.d, c->q.ddoff, qi, &db.d, col0, (int)ncols, ldsc, c->dtab, lq, c->scores, c->mods, opt->contraction, c->st))
+      if (cth_launch_score(&c->q.d, c->q.ddoff, qi, &db.d, col0, (int)ncols, ldsc, c->dtab, lq, scores, mods,
+                           opt->contraction, c->st))
         return -1;
       if (timing) CTH_HIP_CHECK(hipEventRecord(c->ev[1], c->st));
       // phase 1 in two passes: the DP of every profile without traceback,
@@ -195,14 +327,13 @@ extern "C" int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt
         const int b = std::min(p1, a + kDpGroup);
         int lg = 0;
         for (int p = a; p < b; p++) lg = std::max(lg, db.h_len[p]);
-        if (cth_launch_dp_list(&c->q.d, qi, &db.d, a, nullptr, b - a, lg, col0, ldsc, c->scores, nullptr, scorethld,
-                               0, (cth_dpres*)c->dpout + (a - p0), c->st))
+        if (cth_launch_dp_list(&c->q.d, qi, &db.d, a, nullptr, b - a, lg, col0, ldsc, scores, nullptr, scorethld, 0,
+                               dpout + (a - p0), c->st))
           return -1;
       }
       if (timing) CTH_HIP_CHECK(hipEventRecord(c->ev[2], c->st));
       std::vector<cth_dpres> res(p1 - p0);
-      CTH_HIP_CHECK(hipMemcpyAsync(res.data(), c->dpout, sizeof(cth_dpres) * res.size(), hipMemcpyDeviceToHost,
-                                   c->st));
+      CTH_HIP_CHECK(hipMemcpyAsync(res.data(), dpout, sizeof(cth_dpres) * res.size(), hipMemcpyDeviceToHost, c->st));
       CTH_HIP_CHECK(hipStreamSynchronize(c->st));
       if (timing) {
         float a = 0.0f, b = 0.0f;
@@ -224,13 +355,13 @@ extern "C" int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt
       if (!cand.empty()) {
         const int nc = (int)cand.size();
         const size_t wneed = sizeof(int) * nc + sizeof(cth_dpres) * nc + 256;
-        if (cth_ctx_reserve(c, &c->work, &c->cap_work, wneed)) return -1;
-        int* d_cand = (int*)c->work;
+        int* d_cand = (int*)bufs.work(wneed);
+        if (!d_cand) return -1;
         cth_dpres* d_res2 = (cth_dpres*)(((uintptr_t)(d_cand + nc) + 15) & ~(uintptr_t)15);
         CTH_HIP_CHECK(hipMemcpyAsync(d_cand, cand_abs.data(), sizeof(int) * nc, hipMemcpyHostToDevice, c->st));
         if (timing) CTH_HIP_CHECK(hipEventRecord(c->ev[1], c->st));
-        if (cth_launch_dp_list(&c->q.d, qi, &db.d, p0, d_cand, nc, ltc, col0, ldsc, c->scores, c->btck, scorethld,
-                               1, d_res2, c->st))
+        if (cth_launch_dp_list(&c->q.d, qi, &db.d, p0, d_cand, nc, ltc, col0, ldsc, scores, btck, scorethld, 1,
+                               d_res2, c->st))
           return -1;
         if (timing) CTH_HIP_CHECK(hipEventRecord(c->ev[2], c->st));
         std::vector<cth_dpres> res2(nc);
@@ -238,7 +369,7 @@ extern "C" int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt
         CTH_HIP_CHECK(hipStreamSynchronize(c->st));
         for (int k = 0; k < nc; k++) res[cand[k]] = res2[k];
         // the statistics kernel reads the phase-1 records of the chunk from dpout
-        CTH_HIP_CHECK(hipMemcpyAsync(c->dpout, res.data(), sizeof(cth_dpres) * res.size(), hipMemcpyHostToDevice,
+        CTH_HIP_CHECK(hipMemcpyAsync(dpout, res.data(), sizeof(cth_dpres) * res.size(), hipMemcpyHostToDevice,
                                      c->st));
         if (timing) {
           float b = 0.0f;
@@ -255,16 +386,16 @@ extern "C" int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt
       if (np2) {
         const size_t wneed = sizeof(int) * 2 * np2 + sizeof(int) * 641 * (size_t)np2 +
                              sizeof(double) * CTH_STATS_NOUT * (size_t)np2 + 256;
-        if (cth_ctx_reserve(c, &c->work, &c->cap_work, wneed)) return -1;
-        int* d_abs = (int*)c->work;
+        int* d_abs = (int*)bufs.work(wneed);
+        if (!d_abs) return -1;
         int* d_rel = d_abs + np2;
         int* d_hist = d_rel + np2;
         double* d_out = (double*)(((uintptr_t)(d_hist + 641 * (size_t)np2) + 15) & ~(uintptr_t)15);
         CTH_HIP_CHECK(hipMemcpyAsync(d_abs, abs_.data(), sizeof(int) * np2, hipMemcpyHostToDevice, c->st));
         CTH_HIP_CHECK(hipMemcpyAsync(d_rel, rel.data(), sizeof(int) * np2, hipMemcpyHostToDevice, c->st));
         if (timing) CTH_HIP_CHECK(hipEventRecord(c->ev[3], c->st));
-        if (cth_launch_stats(&c->q.d, qi, &db.d, p0, col0, ldsc, c->scores, d_abs, d_rel,
-                             (const cth_dpres*)c->dpout, np2, sspace, opt->ssemodel, d_hist, d_out, c->st))
+        if (cth_launch_stats(&c->q.d, qi, &db.d, p0, col0, ldsc, scores, d_abs, d_rel, dpout, np2, sspace,
+                             opt->ssemodel, d_hist, d_out, c->st))
           return -1;
         if (timing) CTH_HIP_CHECK(hipEventRecord(c->ev[0], c->st));
         CTH_HIP_CHECK(hipMemcpyAsync(sv.data(), d_out, sizeof(double) * sv.size(), hipMemcpyDeviceToHost, c->st));
@@ -293,20 +424,21 @@ extern "C" int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt
           jobs[k].logepa = (float)sv[(size_t)sel[k] * CTH_STATS_NOUT + CTH_ST_LOGEPA];
           ltm = std::max(ltm, db.h_len[jobs[k].profile]);
         }
-        if (cth_ctx_reserve(c, (void**)&c->mapbuf, &c->cap_mapbuf, need)) return -1;
+        float* const post = bufs.map_buffer(lq, ldsc);
+        if (!post) return -1;
         rowcap = lq + ltm;
         const size_t alnbytes = opt->alnbuf ? (size_t)nj * 5 * rowcap : 0;
         const size_t wneed = sizeof(double) * (size_t)nj * lq + sizeof(cth_mapjob) * nj + sizeof(cth_mapres) * nj +
                              alnbytes + 512;
-        if (cth_ctx_reserve(c, &c->work, &c->cap_work, wneed)) return -1;
-        double* d_zrow = (double*)c->work;
+        double* d_zrow = (double*)bufs.work(wneed);
+        if (!d_zrow) return -1;
         cth_mapjob* d_jobs = (cth_mapjob*)(d_zrow + (size_t)nj * lq);
         cth_mapres* d_res = (cth_mapres*)(((uintptr_t)(d_jobs + nj) + 15) & ~(uintptr_t)15);
         char* d_aln = alnbytes ? (char*)(d_res + nj) : nullptr;
         CTH_HIP_CHECK(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(cth_mapjob) * nj, hipMemcpyHostToDevice, c->st));
         if (timing) CTH_HIP_CHECK(hipEventRecord(c->ev[3], c->st));
-        if (cth_launch_map(&c->q.d, qi, lq, &db.d, d_jobs, nj, ltm, col0, ldsc, c->scores, c->mods, c->mapbuf,
-                           c->btck, d_zrow, opt->minpp > 0.0f ? opt->minpp : CTH_MINPP, d_aln, d_res, c->st))
+        if (cth_launch_map(&c->q.d, qi, lq, &db.d, d_jobs, nj, ltm, col0, ldsc, scores, mods, post, btck, d_zrow,
+                           opt->minpp > 0.0f ? opt->minpp : CTH_MINPP, d_aln, d_res, c->st))
           return -1;
         if (timing) CTH_HIP_CHECK(hipEventRecord(c->ev[0], c->st));
         CTH_HIP_CHECK(hipMemcpyAsync(mres.data(), d_res, sizeof(cth_mapres) * nj, hipMemcpyDeviceToHost, c->st));
@@ -334,7 +466,7 @@ extern "C" int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt
         // the host keeps hits with an alignment (CompressResultsPlain,
         // cupro/CuBatchProcessingFinalizerPlain.cpp:87-142)
         if (opt->do_map && mres[k].alnlen < 1) continue;
-        if (nh >= maxhits) { cth_set_error("search: hit buffer full (%d)", maxhits); return -1; }
+        if (nh >= maxhits) { cth_set_error("%s: hit buffer full (%d)", who, maxhits); return -1; }
         cth_hit& h = hits[nh++];
         memset(&h, 0, sizeof h);
         const cth_dpres& r = res[rel[j]];
@@ -366,7 +498,7 @@ extern "C" int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt
             // five rows of alnlen letters each, in reading order
             const size_t n = (size_t)mres[k].alnlen;
             if (alnused + 5 * n > opt->alncap) {
-              cth_set_error("search: alignment buffer full (%llu bytes)", (unsigned long long)opt->alncap);
+              cth_set_error("%s: alignment buffer full (%llu bytes)", who, (unsigned long long)opt->alncap);
               return -1;
             }
             h.alnoff = alnused;
@@ -386,4 +518,26 @@ extern "C" int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt
   }
   if (stats) *stats = st;
   return 0;
+}
+
+}  // namespace
+
+extern "C" int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt, cth_hit* hits, int maxhits,
+                          int* nhits, cth_search_stats* stats) {
+  chunk_bufs bufs(c, nullptr, 0, "search");
+  return search_run(c, q0, q1, opt, hits, maxhits, nhits, stats, bufs);
+}
+
+// The no-allocation variant: the same pipeline with every per-chunk device
+// buffer carved out of this device's fixed scratch array.
+extern "C" int cth_search_noalloc(cth_ctx* c, int q0, int q1, const cth_search_opts* opt, cth_hit* hits, int maxhits,
+                                  int* nhits, cth_search_stats* stats) {
+  if (!c) { cth_set_error("search_noalloc: no context"); return -1; }
+  uint8_t* base = scratch_of(c->device);
+  if (!base) {
+    cth_set_error("search_noalloc: device %d has no scratch address (contexts set it up at creation)", c->device);
+    return -1;
+  }
+  chunk_bufs bufs(c, base, CTH_NOALLOC_SCRATCH_BYTES, "search_noalloc");
+  return search_run(c, q0, q1, opt, hits, maxhits, nhits, stats, bufs);
 }

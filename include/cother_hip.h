@@ -220,6 +220,24 @@ typedef struct cth_search_stats {
 int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt, cth_hit* hits, int maxhits, int* nhits,
                cth_search_stats* stats);
 
+/* As cth_search, with no device allocation.  No counterpart in the
+ * reference: for callers that cannot call hipMalloc (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation).  Every
+ * per-chunk buffer (scores, modular scores, traceback bytes, phase-1
+ * records, MAP posteriors, stage lists) is carved out of one fixed scratch
+ * array of CTH_NOALLOC_SCRATCH_BYTES that each device holds from the first
+ * context created on it until the process ends.  Database chunks take at
+ * most three quarters of it (mem_budget is not used); a query whose chunk of
+ * a single profile does not fit (Lq x Lt x 13 bytes with MAP, 9 without,
+ * about 192 MiB), or a stage whose lists do not fit the remaining quarter,
+ * is refused with an error before any kernel of that chunk runs.  The
+ * results equal cth_search's.  All calls on one device share the array and
+ * it keeps the last call's contents: calls that overlap on one device (from
+ * several threads or contexts) race, so the caller serialises them. */
+#define CTH_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+int cth_search_noalloc(cth_ctx* c, int q0, int q1, const cth_search_opts* opt, cth_hit* hits, int maxhits,
+                       int* nhits, cth_search_stats* stats);
+
 /* Length adjustment and effective search space of a query of qlen positions
  * against dbsize positions in ndb profiles with the gapped parameters
  * (AbstractScoreMatrix::ComputeLengthAdjustment, src/libpro/srcsco/
