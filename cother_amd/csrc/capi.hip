@@ -107,7 +107,7 @@ extern "C" int cth_ctx_create_shared(cth_ctx* parent, cth_ctx** out) {
 extern "C" void cth_ctx_trim(cth_ctx* c, size_t keep) {
   struct { void** p; size_t* cap; } bufs[] = {
       {(void**)&c->scores, &c->cap_scores}, {(void**)&c->mods, &c->cap_mods}, {(void**)&c->btck, &c->cap_btck},
-      {(void**)&c->mapbuf, &c->cap_mapbuf}, {&c->work, &c->cap_work}};
+      {(void**)&c->mapbuf, &c->cap_mapbuf}, {&c->work, &c->cap_work}, {(void**)&c->grow, &c->cap_grow}};
   for (auto& b : bufs)
     if (*b.p && *b.cap > keep) {
       (void)hipFree(*b.p);
@@ -126,7 +126,7 @@ extern "C" void cth_ctx_destroy(cth_ctx* c) {
     cth_devset_free(&c->db);
     if (c->dtab) (void)hipFree(c->dtab);
   }
-  void* bufs[] = {c->scores, c->mods, c->btck, c->dpout, c->work, c->mapbuf};
+  void* bufs[] = {c->scores, c->mods, c->btck, c->dpout, c->work, c->mapbuf, c->grow};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (c->h_pin) (void)hipHostFree(c->h_pin);
@@ -170,6 +170,34 @@ extern "C" int cth_upload_db(cth_ctx* c, const cth_pset* db) {
   return 0;
 }
 
+extern "C" cth_pset* cth_pset_from_packed(char* const* pmbeg, char* const* pmend, const size_t* szfields, int nprof,
+                                          const char* const* descs);
+extern "C" void cth_pset_free(cth_pset* ps);
+
+// The reference's packed chunk buffers become the resident database
+// (CuDeviceMemory::TransferCPMData, cupro/CuDeviceMemory.cu:267-274,
+// 1095-1157, called through CuBatchProcessing::TransferCPMData,
+// cupro/CuBatchProcessing.cuh:62-70).
+extern "C" int cth_upload_db_packed(cth_ctx* c, char* const* pmbeg, char* const* pmend, const size_t* szfields) {
+  cth_pset* db = cth_pset_from_packed(pmbeg, pmend, szfields, -1, nullptr);
+  if (!db) return -1;
+  const int rc = cth_upload_db(c, db);
+  cth_pset_free(db);
+  return rc;
+}
+
+// The query batch from the reference's packed buffers (the query part of
+// CuDeviceMemory::CacheCompleteData, cualn/JobDispatcher.cpp:325-355); nprof
+// as in cth_pset_from_packed.
+extern "C" int cth_upload_queries_packed(cth_ctx* c, char* const* pmbeg, char* const* pmend, const size_t* szfields,
+                                         int nprof) {
+  cth_pset* q = cth_pset_from_packed(pmbeg, pmend, szfields, nprof, nullptr);
+  if (!q) return -1;
+  const int rc = cth_upload_queries(c, q);
+  cth_pset_free(q);
+  return rc;
+}
+
 extern "C" int cth_launch_score(const cth_dpset* q, const int32_t* qoff, int qp, const cth_dpset* db, int64_t col0, int ncols,
                                 int64_t ldsc, const cth_tables* dtabs, int lq, float* scores, float* mods, int contraction,
                                 hipStream_t st);
@@ -208,7 +236,9 @@ extern "C" int cth_score_block(cth_ctx* c, int qidx, int p0, int p1, float* h_sc
 
 extern "C" int cth_launch_dp(const cth_dpset* q, int qp, const cth_dpset* db, int p0, int np, int ltmax,
                              int64_t col0, int64_t ldsc, const float* scores, uint8_t* btck, float scorethld,
-                             cth_dpres* res, hipStream_t st);
+                             cth_dpres* res, float* grow, hipStream_t st);
+extern "C" size_t cth_dp_row_bytes(int np, int ltmax);
+extern "C" size_t cth_map_row_bytes(int njobs, int ltmax);
 
 // Phase 1 (score matrix, DP, finalisation) of query qidx against template
 // profiles [p0,p1); per-profile results to res (p1-p0 records of 6 words:
@@ -233,8 +263,11 @@ extern "C" int cth_phase1_block(cth_ctx* c, int qidx, int p0, int p1, float scor
   if (cth_ctx_reserve(c, (void**)&c->dpout, &c->cap_dpout, sizeof(cth_dpres) * (size_t)(p1 - p0))) return -1;
   if (cth_launch_score(&c->q.d, c->q.ddoff, qidx, &c->db.d, col0, (int)ncols, ldsc, c->dtab, lq, c->scores, c->mods, 0, c->st))
     return -1;
+  // boundary rows in global memory for profiles over the LDS row limit
+  const size_t rows = cth_dp_row_bytes(p1 - p0, ltmax);
+  if (rows && cth_ctx_reserve(c, (void**)&c->grow, &c->cap_grow, rows)) return -1;
   if (cth_launch_dp(&c->q.d, qidx, &c->db.d, p0, p1 - p0, ltmax, col0, ldsc, c->scores, c->btck, scorethld,
-                    (cth_dpres*)c->dpout, c->st))
+                    (cth_dpres*)c->dpout, rows ? c->grow : nullptr, c->st))
     return -1;
   CTH_HIP_CHECK(hipStreamSynchronize(c->st));
   CTH_HIP_CHECK(hipMemcpy(res, c->dpout, sizeof(cth_dpres) * (size_t)(p1 - p0), hipMemcpyDeviceToHost));
@@ -291,7 +324,7 @@ extern "C" int cth_phase2_block(cth_ctx* c, int qidx, int p0, int p1, float scor
 extern "C" int cth_launch_map(const cth_dpset* q, int qp, int lq, const cth_dpset* db, const cth_mapjob* jobs,
                               int njobs, int ltmax, int64_t col0, int64_t ldsc, const float* scores,
                               const float* mods, float* post, uint8_t* btck, double* zrow, float minpp,
-                              char* alnout, cth_mapres* res, hipStream_t st);
+                              char* alnout, int rowcap, cth_mapres* res, float* grow, hipStream_t st);
 
 // Score matrix of query qidx against template profiles [p0,p1), then the MAP
 // stage for njobs of them (absolute indices profiles[], log pair E-values
@@ -333,8 +366,11 @@ extern "C" int cth_map_block(cth_ctx* c, int qidx, int p0, int p1, const int32_t
   CTH_HIP_CHECK(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(cth_mapjob) * njobs, hipMemcpyHostToDevice, c->st));
   if (cth_launch_score(&c->q.d, c->q.ddoff, qidx, &c->db.d, col0, (int)ncols, ldsc, c->dtab, lq, c->scores, c->mods, 0, c->st))
     return -1;
+  const size_t rows = cth_map_row_bytes(njobs, ltmax);
+  if (rows && cth_ctx_reserve(c, (void**)&c->grow, &c->cap_grow, rows)) return -1;
   if (cth_launch_map(&c->q.d, qidx, lq, &c->db.d, d_jobs, njobs, ltmax, col0, ldsc, c->scores, c->mods,
-                     (float*)c->mapbuf, c->btck, d_zrow, minpp, nullptr, d_res, c->st))
+                     (float*)c->mapbuf, c->btck, d_zrow, minpp, nullptr, lq + ltmax, d_res, rows ? c->grow : nullptr,
+                     c->st))
     return -1;
   CTH_HIP_CHECK(hipMemcpyAsync(res, d_res, sizeof(cth_mapres) * njobs, hipMemcpyDeviceToHost, c->st));
   CTH_HIP_CHECK(hipStreamSynchronize(c->st));

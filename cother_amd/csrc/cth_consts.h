@@ -14,6 +14,10 @@
 #define CTH_MAXDST     99   // libpro/srcpro/MOptions.h:15 MAX_N_IR_DISTANCE_VALUES
 #define CTH_DDSTRIDE  128   // bytes per position of the distance list in HBM (n + 99 values + 0xff pad)
 #define CTH_PROSCALE 10000  // libpro/srcpro/PMProfileModel.h:58 ReadWriteScale
+#define CTH_MAXCOLUMNS 50000  // liblib/mylimits.h:50 MAXCOLUMNS: longest profile the readers accept
+// longest profile whose DP/MAP stripe boundary rows (5 floats per column)
+// fit one workgroup's 160 KiB of LDS; longer ones keep them in global memory
+#define CTH_LDS_ROW_MAX 8192
 
 // transition order (cudp/CuBatchDP_com.h:87-96)
 enum { CTH_MM = 0, CTH_MI, CTH_MD, CTH_IM, CTH_II, CTH_DM, CTH_DD };

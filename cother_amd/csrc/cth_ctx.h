@@ -23,6 +23,7 @@ struct cth_ctx {
   float* dpout; size_t cap_dpout;
   void* work; size_t cap_work;
   float* mapbuf; size_t cap_mapbuf;  // MAP posteriors [Lq][ldsc]
+  float* grow; size_t cap_grow;      // DP/MAP stripe boundary rows of profiles over the LDS row limit
   void* h_pin; size_t cap_pin;
   // stage events of cth_search's timing mode (created on first use)
   hipEvent_t ev[4]; int has_ev;

@@ -8,8 +8,12 @@
 // neighbour's state (and the template column's transitions) from lane-1 by a
 // DPP wave_shr:1 move, so the wavefront advances one anti-diagonal per step
 // with no LDS round trip.  The bottom row of a stripe is kept in LDS for the
-// next stripe.  Cell values equal the reference's: rows start at column
-// -1-(q mod 32) with zero boundaries, rows q = 0 mod 32 see zeros above at
+// next stripe (in a global-memory row of the wavefront's own for profiles
+// longer than CTH_LDS_ROW_MAX positions, whose five rows exceed the LDS; the
+// row of column x is read at step x + 31 and overwritten at step x + 95 of the
+// same wavefront, so one row serves both stripes as in LDS).  Cell values
+// equal the reference's: rows start at column -1-(q mod 32) with zero
+// boundaries, rows q = 0 mod 32 see zeros above at
 // negative columns, rows q = 31 mod 32 see zeros left and upper-left at
 // column 0, out-of-range transitions are log(0) = -32768 and cells
 // outside the valid area score -9 -- the boundaries the reference's 32x32
@@ -47,11 +51,16 @@ __device__ __forceinline__ bool in_valid_area(int lq, int lt, int edge, int y, i
   return !corners && 0 <= y && y < lq && 0 <= x && x < lt;
 }
 
+template <bool kGlobalRow>
 __global__ __launch_bounds__(64) void k_dp(cth_dpset q, int qp, cth_dpset db, int pbase,
                                            const int* __restrict__ plist, int64_t col0, int64_t ldsc,
                                            const float* __restrict__ scores, uint8_t* __restrict__ btck,
-                                           cth_dpres* __restrict__ res) {
-  extern __shared__ float bot[];  // [5][ltmax]: bottom row of the previous stripe
+                                           cth_dpres* __restrict__ res, float* __restrict__ grow,
+                                           int64_t growstride) {
+  // [5][lt]: bottom row of the previous stripe, in LDS or in this
+  // wavefront's growstride floats of grow
+  extern __shared__ float lds_row[];
+  float* const bot = kGlobalRow ? grow + (int64_t)blockIdx.x * growstride : lds_row;
   const int lane = threadIdx.x;
   const int p = plist ? plist[blockIdx.x] : pbase + blockIdx.x;
   const int lq = q.len[qp], lt = db.len[p];
@@ -59,7 +68,7 @@ __global__ __launch_bounds__(64) void k_dp(cth_dpset q, int qp, cth_dpset db, in
   const int64_t qb = q.dist[qp] + qp;      // query transition vector 0
   const int64_t tb = db.dist[p] + p;       // template transition vector 0
   const int edge = corner_maxlen(lq, lt);
-  const int ltmax = lt;  // row stride of bot (the launch reserves 5 x the longest profile)
+  const int ltmax = lt;  // row stride of bot (the launch reserves 5 x its longest profile per wavefront)
   float best = 0.0f;
   uint32_t bestxy = 0;
   const int nstripes = (lq + 63) >> 6;
@@ -281,18 +290,33 @@ extern "C" int cth_selftest_dpp(int* host_out64) {
   return 0;
 }
 
+// Bytes of global boundary rows a launch over np profiles of at most ltmax
+// positions needs (0 when the rows fit the LDS).
+extern "C" size_t cth_dp_row_bytes(int np, int ltmax) {
+  return ltmax > CTH_LDS_ROW_MAX ? sizeof(float) * 5 * (size_t)ltmax * (size_t)(np > 0 ? np : 0) : 0;
+}
+
 // Phase 1 for np profiles: p0 + k, or plist[k] (device array of absolute
 // indices) when plist is not NULL; res[k] receives profile k's record.
 // btck NULL: no traceback bytes (maximum and its coordinates only);
-// finalize: run the finalisation (needs btck).
+// finalize: run the finalisation (needs btck).  grow: cth_dp_row_bytes(np,
+// ltmax) bytes of device memory when ltmax exceeds CTH_LDS_ROW_MAX (else
+// unused).
 extern "C" int cth_launch_dp_list(const cth_dpset* q, int qp, const cth_dpset* db, int p0, const int* plist, int np,
                                   int ltmax, int64_t col0, int64_t ldsc, const float* scores, uint8_t* btck,
-                                  float scorethld, int finalize, cth_dpres* res, hipStream_t st) {
+                                  float scorethld, int finalize, cth_dpres* res, float* grow, hipStream_t st) {
   if (np <= 0) return 0;
   if (finalize && !btck) { cth_set_error("dp: finalisation needs the traceback"); return -1; }
-  const size_t lds = sizeof(float) * 5 * (size_t)(ltmax > 0 ? ltmax : 1);
-  if (lds > 160 * 1024) { cth_set_error("dp: profile length %d exceeds the LDS budget", ltmax); return -1; }
-  hipLaunchKernelGGL(k_dp, dim3(np), dim3(64), lds, st, *q, qp, *db, p0, plist, col0, ldsc, scores, btck, res);
+  if (ltmax > CTH_MAXCOLUMNS) { cth_set_error("dp: profile length %d over %d", ltmax, CTH_MAXCOLUMNS); return -1; }
+  if (ltmax > CTH_LDS_ROW_MAX) {
+    if (!grow) { cth_set_error("dp: profile length %d needs global boundary rows", ltmax); return -1; }
+    hipLaunchKernelGGL(k_dp<true>, dim3(np), dim3(64), 0, st, *q, qp, *db, p0, plist, col0, ldsc, scores, btck, res,
+                       grow, (int64_t)5 * ltmax);
+  } else {
+    const size_t lds = sizeof(float) * 5 * (size_t)(ltmax > 0 ? ltmax : 1);
+    hipLaunchKernelGGL(k_dp<false>, dim3(np), dim3(64), lds, st, *q, qp, *db, p0, plist, col0, ldsc, scores, btck,
+                       res, nullptr, (int64_t)0);
+  }
   CTH_HIP_CHECK(hipGetLastError());
   if (finalize) {
     hipLaunchKernelGGL(k_dpfin, dim3(np), dim3(64), 0, st, *db, p0, plist, col0, ldsc, scores, btck, scorethld, res);
@@ -303,6 +327,6 @@ extern "C" int cth_launch_dp_list(const cth_dpset* q, int qp, const cth_dpset* d
 
 extern "C" int cth_launch_dp(const cth_dpset* q, int qp, const cth_dpset* db, int p0, int np, int ltmax,
                              int64_t col0, int64_t ldsc, const float* scores, uint8_t* btck, float scorethld,
-                             cth_dpres* res, hipStream_t st) {
-  return cth_launch_dp_list(q, qp, db, p0, nullptr, np, ltmax, col0, ldsc, scores, btck, scorethld, 1, res, st);
+                             cth_dpres* res, float* grow, hipStream_t st) {
+  return cth_launch_dp_list(q, qp, db, p0, nullptr, np, ltmax, col0, ldsc, scores, btck, scorethld, 1, res, grow, st);
 }

@@ -16,7 +16,10 @@
 // MI355X layout as in phase 1: the 64 lanes of a wavefront own 64
 // consecutive query rows and sweep the anti-diagonals of the stripe, taking
 // the neighbour row's state by a DPP row move (wave_shr:1 forward,
-// wave_shl:1 backward); the boundary row between stripes sits in LDS.
+// wave_shl:1 backward); the boundary row between stripes sits in LDS, or,
+// for profiles longer than CTH_LDS_ROW_MAX positions, in a global-memory row
+// of the wavefront's own (each column's entry is read before the same
+// wavefront overwrites it for the next stripe, as in LDS).
 // Boundary values are those of the reference's 32x32 oblique block schedule:
 // forward rows start at column -1-(q mod 32) with log(0) = -99 around them,
 // rows q = 0 mod 32 see -99 above at negative columns, rows q = 31 mod 32
@@ -111,11 +114,15 @@ __device__ __forceinline__ void trn_bwd(const float* __restrict__ trn, int64_t l
   for (int k = 0; k < NT; k++) t[k] = trn[k * ldtr + b + 1];
 }
 
+template <bool kGlobalRow>
 __global__ __launch_bounds__(64) void k_map_fwd(cth_dpset q, int qp, cth_dpset db, const cth_mapjob* __restrict__ jobs,
                                                 int64_t col0, int64_t ldsc, const float* __restrict__ scores,
                                                 const float* __restrict__ mods, float* __restrict__ post,
-                                                cth_mapres* __restrict__ res) {
-  extern __shared__ float bot[];  // [5][lt]: bottom row of the previous stripe
+                                                cth_mapres* __restrict__ res, float* __restrict__ grow,
+                                                int64_t growstride) {
+  // [5][lt]: bottom row of the previous stripe (LDS, or this wavefront's part of grow)
+  extern __shared__ float lds_row[];
+  float* const bot = kGlobalRow ? grow + (int64_t)blockIdx.x * growstride : lds_row;
   const int lane = threadIdx.x;
   const cth_mapjob job = jobs[blockIdx.x];
   const int p = job.profile;
@@ -194,11 +201,15 @@ __global__ __launch_bounds__(64) void k_map_fwd(cth_dpset q, int qp, cth_dpset d
   if (lane == 0) res[blockIdx.x].maxlog = best;
 }
 
+template <bool kGlobalRow>
 __global__ __launch_bounds__(64) void k_map_bwd(cth_dpset q, int qp, cth_dpset db, const cth_mapjob* __restrict__ jobs,
                                                 int64_t col0, int64_t ldsc, const float* __restrict__ scores,
                                                 const float* __restrict__ mods, float* __restrict__ post,
-                                                double* __restrict__ zrow, int64_t zld, cth_mapres* __restrict__ res) {
-  extern __shared__ float top[];  // [5][lt]: top row of the stripe below
+                                                double* __restrict__ zrow, int64_t zld, cth_mapres* __restrict__ res,
+                                                float* __restrict__ grow, int64_t growstride) {
+  // [5][lt]: top row of the stripe below (LDS, or this wavefront's part of grow)
+  extern __shared__ float lds_row[];
+  float* const top = kGlobalRow ? grow + (int64_t)blockIdx.x * growstride : lds_row;
   const int lane = threadIdx.x;
   const cth_mapjob job = jobs[blockIdx.x];
   const int p = job.profile;
@@ -274,11 +285,15 @@ __global__ __launch_bounds__(64) void k_map_bwd(cth_dpset q, int qp, cth_dpset d
   }
 }
 
+template <bool kGlobalRow>
 __global__ __launch_bounds__(64) void k_map_align(cth_dpset q, int qp, cth_dpset db,
                                                   const cth_mapjob* __restrict__ jobs, int64_t col0, int64_t ldsc,
                                                   const float* __restrict__ post, uint8_t* __restrict__ btck,
-                                                  float minpp, cth_mapres* __restrict__ res) {
-  extern __shared__ float bot[];  // [lt]: bottom row of the previous stripe
+                                                  float minpp, cth_mapres* __restrict__ res, float* __restrict__ grow,
+                                                  int64_t growstride) {
+  // [lt]: bottom row of the previous stripe (LDS, or this wavefront's part of grow)
+  extern __shared__ float lds_row[];
+  float* const bot = kGlobalRow ? grow + (int64_t)blockIdx.x * growstride : lds_row;
   const int lane = threadIdx.x;
   const int p = jobs[blockIdx.x].profile;
   const int lq = q.len[qp], lt = db.len[p];
@@ -411,29 +426,51 @@ extern "C" int cth_selftest_dpp_shl(int* host_out64) {
   return 0;
 }
 
+// Bytes of global boundary rows a MAP launch over njobs profiles of at most
+// ltmax positions needs (0 when the rows fit the LDS).
+extern "C" size_t cth_map_row_bytes(int njobs, int ltmax) {
+  return ltmax > CTH_LDS_ROW_MAX ? sizeof(float) * 5 * (size_t)ltmax * (size_t)(njobs > 0 ? njobs : 0) : 0;
+}
+
 // The MAP stage for njobs profiles of the chunk starting at column col0
 // (query qp of length lq; ltmax the longest job profile): post [lq][ldsc]
 // floats of scratch, btck [lq][ldsc] bytes (overwritten for the job
 // profiles), zrow njobs x lq doubles of scratch, alnout (optional) njobs x 5
-// x (lq + ltmax) bytes for the alignment rows, res one record per job.
+// x rowcap bytes for the alignment rows (rowcap >= lq + ltmax), res one
+// record per job; grow cth_map_row_bytes(njobs, ltmax) bytes of device
+// memory when ltmax exceeds CTH_LDS_ROW_MAX (else unused).
 extern "C" int cth_launch_map(const cth_dpset* q, int qp, int lq, const cth_dpset* db, const cth_mapjob* jobs,
                               int njobs, int ltmax, int64_t col0, int64_t ldsc, const float* scores,
                               const float* mods, float* post, uint8_t* btck, double* zrow, float minpp,
-                              char* alnout, cth_mapres* res, hipStream_t st) {
+                              char* alnout, int rowcap, cth_mapres* res, float* grow, hipStream_t st) {
   if (njobs <= 0) return 0;
-  const size_t lds5 = sizeof(float) * 5 * (size_t)(ltmax > 0 ? ltmax : 1);
-  if (lds5 > 160 * 1024) { cth_set_error("map: profile length %d exceeds the LDS budget", ltmax); return -1; }
-  hipLaunchKernelGGL(k_map_fwd, dim3(njobs), dim3(64), lds5, st, *q, qp, *db, jobs, col0, ldsc, scores, mods, post,
-                     res);
-  CTH_HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_map_bwd, dim3(njobs), dim3(64), lds5, st, *q, qp, *db, jobs, col0, ldsc, scores, mods, post,
-                     zrow, (int64_t)lq, res);
-  CTH_HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_map_align, dim3(njobs), dim3(64), sizeof(float) * (size_t)(ltmax > 0 ? ltmax : 1), st, *q, qp,
-                     *db, jobs, col0, ldsc, post, btck, minpp, res);
+  if (ltmax > CTH_MAXCOLUMNS) { cth_set_error("map: profile length %d over %d", ltmax, CTH_MAXCOLUMNS); return -1; }
+  if (alnout && rowcap < lq + ltmax) { cth_set_error("map: row capacity %d < %d", rowcap, lq + ltmax); return -1; }
+  if (ltmax > CTH_LDS_ROW_MAX) {
+    if (!grow) { cth_set_error("map: profile length %d needs global boundary rows", ltmax); return -1; }
+    const int64_t gs = (int64_t)5 * ltmax;
+    hipLaunchKernelGGL(k_map_fwd<true>, dim3(njobs), dim3(64), 0, st, *q, qp, *db, jobs, col0, ldsc, scores, mods,
+                       post, res, grow, gs);
+    CTH_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_map_bwd<true>, dim3(njobs), dim3(64), 0, st, *q, qp, *db, jobs, col0, ldsc, scores, mods,
+                       post, zrow, (int64_t)lq, res, grow, gs);
+    CTH_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_map_align<true>, dim3(njobs), dim3(64), 0, st, *q, qp, *db, jobs, col0, ldsc, post, btck,
+                       minpp, res, grow, gs);
+  } else {
+    const size_t lds5 = sizeof(float) * 5 * (size_t)(ltmax > 0 ? ltmax : 1);
+    hipLaunchKernelGGL(k_map_fwd<false>, dim3(njobs), dim3(64), lds5, st, *q, qp, *db, jobs, col0, ldsc, scores,
+                       mods, post, res, nullptr, (int64_t)0);
+    CTH_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_map_bwd<false>, dim3(njobs), dim3(64), lds5, st, *q, qp, *db, jobs, col0, ldsc, scores,
+                       mods, post, zrow, (int64_t)lq, res, nullptr, (int64_t)0);
+    CTH_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_map_align<false>, dim3(njobs), dim3(64), sizeof(float) * (size_t)(ltmax > 0 ? ltmax : 1),
+                       st, *q, qp, *db, jobs, col0, ldsc, post, btck, minpp, res, nullptr, (int64_t)0);
+  }
   CTH_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_map_fin, dim3(njobs), dim3(64), 0, st, *q, qp, *db, jobs, col0, ldsc, scores, btck, alnout,
-                     lq + ltmax, res);
+                     rowcap, res);
   CTH_HIP_CHECK(hipGetLastError());
   return 0;
 }

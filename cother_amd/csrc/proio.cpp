@@ -379,9 +379,73 @@ extern "C" int cth_write_text_profiles(const cth_pset* ps, const char* path, int
   return 0;
 }
 
+namespace {
+
+// Field indices of the reference's packed profile model, TPM2DVectorFields
+// (cupro/PM2DVectorFields.h:53-87): 181 fields, one array each, over
+// profiles (background probabilities, ENO, length, offset), over the
+// npos + nprof transition vectors, or over positions.
+enum {
+  fBkg = 0, fENO = fBkg + CTH_NAA, fLen, fDist, fTrn, fTrg = fTrn + CTH_NTRN, fCV = fTrg + CTH_NAA,
+  fCVprior = fCV + CTH_NCV, fCVnorm2, fDD, fSS = fDD + kMaxDst + 1, fHDPprb = fSS + 3, fHDPind, fAddrPro, fAddrCV,
+  fAddrSS, fAA, fSSstate, fTot
+};
+static_assert(fTot == 181, "pmv2DTotFlds");
+
+// A profile set from per-field arrays in the packed layout (f[k]: field k's
+// first element; lengths already checked to be >= 1 and to sum to npos).
+cth_pset* decode_fields(const char* const* f, size_t nprof, size_t npos, const int32_t* lens) {
+  cth_pset* o = cth_pset_alloc((int)nprof, lens);
+  if (!o) return nullptr;
+  auto f32 = [&](int k, size_t i) {
+    float v;
+    memcpy(&v, f[k] + 4 * i, 4);
+    return v;
+  };
+  const size_t ntr = npos + nprof;
+  for (size_t p = 0; p < nprof; p++) {
+    o->eno[p] = f32(fENO, p);
+    for (int r = 0; r < CTH_NAA; r++) o->bkg[p * CTH_NAA + r] = f32(fBkg + r, p);
+  }
+  for (int t = 0; t < CTH_NTRN; t++)
+    for (size_t k = 0; k < ntr; k++) o->trn[t * ntr + k] = f32(fTrn + t, k);
+  for (size_t i = 0; i < npos; i++) {
+    for (int r = 0; r < CTH_NAA; r++) o->trg[r * npos + i] = f32(fTrg + r, i);
+    for (int k = 0; k < CTH_NCV; k++) o->cv[k * npos + i] = f32(fCV + k, i);
+    o->cvprior[i] = f32(fCVprior, i);
+    o->cvnorm2[i] = f32(fCVnorm2, i);
+    for (int q = 0; q < 3; q++) o->ss[q * npos + i] = f32(fSS + q, i);
+    o->aa[i] = f[fAA][i];
+    o->sss[i] = f[fSSstate][i];
+    uint8_t* rec = o->dd + i * CTH_DDSTRIDE;
+    memset(rec, 255, CTH_DDSTRIDE);
+    const int nd = (int)f32(fDD, i);
+    rec[0] = (uint8_t)(nd < 0 ? 0 : (nd > kMaxDst ? kMaxDst : nd));
+    for (int k = 0; k < rec[0]; k++) {
+      const float d = f32(fDD + 1 + k, i);
+      rec[1 + k] = (uint8_t)(d < 0.f ? 0 : (d > 255.f ? 255 : (int)d));
+    }
+  }
+  // a profile without SS predictions has blank SS states and zero probabilities
+  for (size_t p = 0; p < nprof; p++) {
+    const int64_t d0 = o->dist[p];
+    bool any = false;
+    for (int m = 0; m < o->len[p] && !any; m++)
+      any = o->ss[0 * npos + d0 + m] != 0.f || o->ss[1 * npos + d0 + m] != 0.f || o->ss[2 * npos + d0 + m] != 0.f;
+    if (!any)
+      for (int m = 0; m < o->len[p]; m++) o->sss[d0 + m] = 0;
+  }
+  return o;
+}
+
+}  // namespace
+
 // Read a binary profile database ("COTHER profile bin database v2.5", layout
 // v2.2: Db::WriteBinaryHeader, libpro/srcpro/Db.cpp:708-717, then the
 // address table of CalculateAndWriteAddressTable_v2_2 and the sections).
+// Its sections hold the packed fields' arrays, which the reference's reader
+// copies into the chunk buffer as they are (TdDataReader::ReadDataChunkHelper,
+// libmycu/cupro/TdDataReader.cpp:703-925).
 extern "C" cth_pset* cth_read_binary_db(const char* path) {
   FILE* fp = fopen(path, "rb");
   if (!fp) { cth_set_error("cannot open %s", path); return nullptr; }
@@ -437,36 +501,23 @@ extern "C" cth_pset* cth_read_binary_db(const char* path) {
     tot += (size_t)l;
   }
   if (tot != npos) { cth_set_error("%s: lengths sum to %zu, header says %zu", path, tot, npos); return nullptr; }
-  cth_pset* o = cth_pset_alloc((int)nprof, lens.data());
+  // the packed fields' arrays in the file (the position-address fields are
+  // not stored: the reader fills them, TdDataReader.cpp:953-978)
+  const char* f[fTot] = {};
+  const char* b = buf.data();
+  for (int r = 0; r < CTH_NAA; r++) { f[fBkg + r] = b + addr[sBkg + r]; f[fTrg + r] = b + addr[sTrg + r]; }
+  f[fENO] = b + addr[sENO];
+  f[fLen] = b + addr[sLen];
+  for (int t = 0; t < CTH_NTRN; t++) f[fTrn + t] = b + addr[sTrn + t];
+  for (int k = 0; k < CTH_NCV; k++) f[fCV + k] = b + addr[sCV + k];
+  f[fCVprior] = b + addr[sCVprior];
+  f[fCVnorm2] = b + addr[sCVnorm2];
+  for (int k = 0; k <= kMaxDst; k++) f[fDD + k] = b + addr[sDst + k];
+  for (int q = 0; q < 3; q++) f[fSS + q] = b + addr[sSS + q];
+  f[fAA] = b + addr[sAA];
+  f[fSSstate] = b + addr[sSSstate];
+  cth_pset* o = decode_fields(f, nprof, npos, lens.data());
   if (!o) return nullptr;
-  auto f32 = [&](int s, size_t k) {
-    float v;
-    memcpy(&v, buf.data() + addr[s] + 4 * k, 4);
-    return v;
-  };
-  for (size_t p = 0; p < nprof; p++) {
-    o->eno[p] = f32(sENO, p);
-    for (int r = 0; r < CTH_NAA; r++) o->bkg[p * CTH_NAA + r] = f32(sBkg + r, p);
-  }
-  for (int t = 0; t < CTH_NTRN; t++)
-    for (size_t k = 0; k < ntr; k++) o->trn[t * ntr + k] = f32(sTrn + t, k);
-  for (size_t i = 0; i < npos; i++) {
-    for (int r = 0; r < CTH_NAA; r++) o->trg[r * npos + i] = f32(sTrg + r, i);
-    for (int k = 0; k < CTH_NCV; k++) o->cv[k * npos + i] = f32(sCV + k, i);
-    o->cvprior[i] = f32(sCVprior, i);
-    o->cvnorm2[i] = f32(sCVnorm2, i);
-    for (int q = 0; q < 3; q++) o->ss[q * npos + i] = f32(sSS + q, i);
-    o->aa[i] = buf[addr[sAA] + i];
-    o->sss[i] = buf[addr[sSSstate] + i];
-    uint8_t* rec = o->dd + i * CTH_DDSTRIDE;
-    memset(rec, 255, CTH_DDSTRIDE);
-    const int nd = (int)f32(sDst, i);
-    rec[0] = (uint8_t)(nd < 0 ? 0 : (nd > kMaxDst ? kMaxDst : nd));
-    for (int k = 0; k < rec[0]; k++) {
-      const float d = f32(sDst + 1 + k, i);
-      rec[1 + k] = (uint8_t)(d < 0.f ? 0 : (d > 255.f ? 255 : (int)d));
-    }
-  }
   // descriptions: NUL-terminated strings from addr[sDesc] on, profile k ending
   // (after its NUL) at the file address held in entry k of the end-address section
   if (need(sDescEnd, 8 * nprof)) {
@@ -480,14 +531,88 @@ extern "C" cth_pset* cth_read_binary_db(const char* path) {
       beg = end;
     }
   }
-  // a profile without SS predictions has blank SS states and zero probabilities
-  for (size_t p = 0; p < nprof; p++) {
-    const int64_t d0 = o->dist[p];
-    bool any = false;
-    for (int m = 0; m < o->len[p] && !any; m++)
-      any = o->ss[0 * npos + d0 + m] != 0.f || o->ss[1 * npos + d0 + m] != 0.f || o->ss[2 * npos + d0 + m] != 0.f;
-    if (!any)
-      for (int m = 0; m < o->len[p]; m++) o->sss[d0 + m] = 0;
-  }
   return o;
 }
+
+// A profile set from the reference's packed profile-model buffers: pmbeg[k]
+// points at field k's array (181 fields, cupro/PM2DVectorFields.h:53-87, as
+// TdDataReader::ReadDataChunkHelper fills them for a database chunk,
+// libmycu/cupro/TdDataReader.cpp:703-980, and PMBatchProData for the query
+// batch).  nprof < 0: every profile, counted from pmend (the end of each
+// field's array); nprof >= 0: the first nprof profiles from pmbeg, as a
+// worker reads one query after PMBatchProData::PMDataSkipNPros has advanced
+// the pointers (cualn/DevCommThread.cpp:314-320; pmend may then be NULL, and
+// when given must bound at least those profiles).  szfields (optional)
+// holds each field's byte offset from pmbeg[0] (szCpm2dvfields), the offsets
+// at which CuDeviceMemory::CopyCPMDataToDevice addresses the fields of the
+// one contiguous buffer (cupro/CuDeviceMemory.cu:1095-1141); they are
+// checked against pmbeg.  descs (optional) holds one description per profile.
+extern "C" cth_pset* cth_pset_from_packed(char* const* pmbeg, char* const* pmend, const size_t* szfields, int nprof,
+                                          const char* const* descs) {
+  if (!pmbeg || (nprof < 0 && !pmend)) { cth_set_error("packed: null field pointers"); return nullptr; }
+  for (int k = 0; k < fTot; k++)
+    if (!pmbeg[k] || (pmend && (!pmend[k] || pmend[k] < pmbeg[k]))) {
+      cth_set_error("packed: field %d unset", k);
+      return nullptr;
+    }
+  if (szfields)
+    for (int k = 0; k < fTot; k++)
+      if (pmbeg[k] != pmbeg[0] + szfields[k]) {
+        cth_set_error("packed: field %d at offset %lld, szfields says %zu", k, (long long)(pmbeg[k] - pmbeg[0]),
+                      szfields[k]);
+        return nullptr;
+      }
+  auto bytes = [&](int k) { return (size_t)(pmend[k] - pmbeg[k]); };
+  size_t np = 0;
+  if (nprof < 0) {
+    if (bytes(fLen) % 4) { cth_set_error("packed: length field of %zu bytes", bytes(fLen)); return nullptr; }
+    np = bytes(fLen) / 4;
+  } else {
+    np = (size_t)nprof;
+  }
+  if (np > (size_t)INT32_MAX) { cth_set_error("packed: %zu profiles", np); return nullptr; }
+  if (pmend && bytes(fLen) < 4 * np) { cth_set_error("packed: %zu profiles beyond the length field", np); return nullptr; }
+  std::vector<int32_t> lens(np);
+  if (np) memcpy(lens.data(), pmbeg[fLen], 4 * np);
+  size_t npos = 0;
+  uint32_t d0 = 0;
+  for (size_t p = 0; p < np; p++) {
+    if (lens[p] < 1 || lens[p] > CTH_MAXCOLUMNS) {
+      cth_set_error("packed: profile %zu has length %d", p, (int)lens[p]);
+      return nullptr;
+    }
+    // the offsets count positions from the first profile of the buffer
+    uint32_t d;
+    memcpy(&d, pmbeg[fDist] + 4 * p, 4);
+    if (p == 0) d0 = d;
+    if (d - d0 != npos) {
+      cth_set_error("packed: profile %zu at position %u, the lengths give %zu", p, d - d0, npos);
+      return nullptr;
+    }
+    npos += (size_t)lens[p];
+  }
+  // each field's size (FPTYPE, INTYPE and LNTYPE are 4 bytes, CHTYPE 1)
+  if (pmend)
+    for (int k = 0; k < fTot; k++) {
+      size_t want = 4 * npos;
+      if (k < fTrn) want = 4 * np;
+      else if (k < fTrg) want = 4 * (npos + np);
+      else if (k == fAA || k == fSSstate) want = npos;
+      if (nprof < 0 ? bytes(k) != want : bytes(k) < want) {
+        cth_set_error("packed: field %d holds %zu bytes, %zu profiles of %zu positions need %zu", k, bytes(k), np,
+                      npos, want);
+        return nullptr;
+      }
+    }
+  const char* f[fTot];
+  for (int k = 0; k < fTot; k++) f[k] = pmbeg[k];
+  cth_pset* o = decode_fields(f, np, npos, lens.data());
+  if (!o) return nullptr;
+  if (descs)
+    for (size_t p = 0; p < np; p++)
+      if (cth_pset_set_text(o, (int)p, nullptr, descs[p])) { cth_pset_free(o); return nullptr; }
+  return o;
+}
+
+extern "C" int cth_pset_nprof(const cth_pset* ps) { return ps ? ps->nprof : 0; }
+extern "C" int64_t cth_pset_npos(const cth_pset* ps) { return ps ? ps->npos : 0; }

@@ -36,7 +36,9 @@ extern "C" int cth_launch_score(const cth_dpset* q, const int32_t* qoff, int qp,
                                 hipStream_t st);
 extern "C" int cth_launch_dp_list(const cth_dpset* q, int qp, const cth_dpset* db, int p0, const int* plist, int np,
                                   int ltmax, int64_t col0, int64_t ldsc, const float* scores, uint8_t* btck,
-                                  float scorethld, int finalize, cth_dpres* res, hipStream_t st);
+                                  float scorethld, int finalize, cth_dpres* res, float* grow, hipStream_t st);
+extern "C" size_t cth_dp_row_bytes(int np, int ltmax);
+extern "C" size_t cth_map_row_bytes(int njobs, int ltmax);
 extern "C" int cth_launch_stats(const cth_dpset* q, int qp, const cth_dpset* db, int p0, int64_t col0, int64_t ldsc,
                                 const float* scores, const int* plist_abs, const int* plist_rel,
                                 const cth_dpres* dpres, int np2, double searchspace, int ssemodel, int* hist,
@@ -44,7 +46,7 @@ extern "C" int cth_launch_stats(const cth_dpset* q, int qp, const cth_dpset* db,
 extern "C" int cth_launch_map(const cth_dpset* q, int qp, int lq, const cth_dpset* db, const cth_mapjob* jobs,
                               int njobs, int ltmax, int64_t col0, int64_t ldsc, const float* scores,
                               const float* mods, float* post, uint8_t* btck, double* zrow, float minpp,
-                              char* alnout, cth_mapres* res, hipStream_t st);
+                              char* alnout, int rowcap, cth_mapres* res, float* grow, hipStream_t st);
 
 namespace {
 
@@ -195,6 +197,22 @@ struct chunk_bufs {
     return mapbuf = c->mapbuf;
   }
 
+  // How many profiles of one launch get global boundary rows of per_item
+  // bytes each when the stage's lists take `used` bytes of the work area (at
+  // most `want`; 0 with the error set when not even one fits).  Growing
+  // buffers hold up to kRowBudget of rows per launch.
+  int row_batch(size_t per_item, size_t used, int want) {
+    constexpr size_t kRowBudget = (size_t)1 << 30;
+    size_t room = kRowBudget;
+    if (fixed) room = fixed_cap - work_off > used + 256 ? fixed_cap - work_off - used - 256 : 0;
+    const size_t n = per_item ? room / per_item : (size_t)want;
+    if (n < 1) {
+      cth_set_error("%s: boundary rows of %zu bytes per profile do not fit the scratch", who, per_item);
+      return 0;
+    }
+    return (int)std::min<size_t>(n, (size_t)want);
+  }
+
   // the work area of a stage (null with the error set when it does not fit)
   void* work(size_t need) {
     if (!fixed) {
@@ -322,14 +340,27 @@ int search_run(cth_ctx* c, int q0, int q1, const cth_search_opts* opt, cth_hit* 
       // in groups of consecutive profiles, each launch reserving LDS for its
       // own longest profile, so that groups of shorter profiles (databases
       // are sorted by length) run more wavefronts per CU.
+      // Profiles longer than CTH_LDS_ROW_MAX keep their stripe boundary rows
+      // in the work area, in launches of as many profiles as their rows fit.
       constexpr int kDpGroup = 2048;
       for (int a = p0; a < p1; a += kDpGroup) {
         const int b = std::min(p1, a + kDpGroup);
         int lg = 0;
         for (int p = a; p < b; p++) lg = std::max(lg, db.h_len[p]);
-        if (cth_launch_dp_list(&c->q.d, qi, &db.d, a, nullptr, b - a, lg, col0, ldsc, scores, nullptr, scorethld, 0,
-                               dpout + (a - p0), c->st))
-          return -1;
+        int nb = b - a;
+        float* grow = nullptr;
+        if (lg > CTH_LDS_ROW_MAX) {
+          nb = bufs.row_batch(cth_dp_row_bytes(1, lg), 0, b - a);
+          if (nb < 1) return -1;
+          grow = (float*)bufs.work(cth_dp_row_bytes(nb, lg));
+          if (!grow) return -1;
+        }
+        for (int a2 = a; a2 < b; a2 += nb) {
+          const int b2 = std::min(b, a2 + nb);
+          if (cth_launch_dp_list(&c->q.d, qi, &db.d, a2, nullptr, b2 - a2, lg, col0, ldsc, scores, nullptr, scorethld,
+                                 0, dpout + (a2 - p0), grow, c->st))
+            return -1;
+        }
       }
       if (timing) CTH_HIP_CHECK(hipEventRecord(c->ev[2], c->st));
       std::vector<cth_dpres> res(p1 - p0);
@@ -342,27 +373,50 @@ int search_run(cth_ctx* c, int q0, int q1, const cth_search_opts* opt, cth_hit* 
         st.ms_k_score += a;
         st.ms_k_dp += b;
       }
-      std::vector<int> cand, cand_abs;
-      int ltc = 0;
+      // candidates for the traceback pass: profiles within the LDS row limit
+      // first, then the longer ones (global boundary rows, in batches)
+      std::vector<int> cand, cand_abs, cand_long;
+      int ltc = 0, ltl = 0;
       for (int k = 0; k < p1 - p0; k++) {
         res[k].passed = 0; res[k].score = 0.0f; res[k].psts = 0; res[k].beg = 0;
         if (scorethld < res[k].maxsc) {
-          cand.push_back(k);
-          cand_abs.push_back(p0 + k);
-          ltc = std::max(ltc, db.h_len[p0 + k]);
+          const int lt = db.h_len[p0 + k];
+          if (lt > CTH_LDS_ROW_MAX) {
+            cand_long.push_back(k);
+            ltl = std::max(ltl, lt);
+          } else {
+            cand.push_back(k);
+            cand_abs.push_back(p0 + k);
+            ltc = std::max(ltc, lt);
+          }
         }
       }
+      const int ns = (int)cand.size();
+      for (int k : cand_long) {
+        cand.push_back(k);
+        cand_abs.push_back(p0 + k);
+      }
       if (!cand.empty()) {
-        const int nc = (int)cand.size();
-        const size_t wneed = sizeof(int) * nc + sizeof(cth_dpres) * nc + 256;
-        int* d_cand = (int*)bufs.work(wneed);
+        const int nc = (int)cand.size(), nl = nc - ns;
+        const size_t lists = ((sizeof(int) * nc + 15) & ~(size_t)15) + sizeof(cth_dpres) * nc + 256;
+        int nbl = nl;
+        if (nl) {
+          nbl = bufs.row_batch(cth_dp_row_bytes(1, ltl), lists, nl);
+          if (nbl < 1) return -1;
+        }
+        int* d_cand = (int*)bufs.work(lists + (nl ? cth_dp_row_bytes(nbl, ltl) : 0));
         if (!d_cand) return -1;
         cth_dpres* d_res2 = (cth_dpres*)(((uintptr_t)(d_cand + nc) + 15) & ~(uintptr_t)15);
+        float* grow = nl ? (float*)(((uintptr_t)(d_res2 + nc) + 255) & ~(uintptr_t)255) : nullptr;
         CTH_HIP_CHECK(hipMemcpyAsync(d_cand, cand_abs.data(), sizeof(int) * nc, hipMemcpyHostToDevice, c->st));
         if (timing) CTH_HIP_CHECK(hipEventRecord(c->ev[1], c->st));
-        if (cth_launch_dp_list(&c->q.d, qi, &db.d, p0, d_cand, nc, ltc, col0, ldsc, scores, btck, scorethld, 1,
-                               d_res2, c->st))
+        if (cth_launch_dp_list(&c->q.d, qi, &db.d, p0, d_cand, ns, ltc, col0, ldsc, scores, btck, scorethld, 1,
+                               d_res2, nullptr, c->st))
           return -1;
+        for (int j = 0; j < nl; j += nbl)
+          if (cth_launch_dp_list(&c->q.d, qi, &db.d, p0, d_cand + ns + j, std::min(nbl, nl - j), ltl, col0, ldsc,
+                                 scores, btck, scorethld, 1, d_res2 + ns + j, grow, c->st))
+            return -1;
         if (timing) CTH_HIP_CHECK(hipEventRecord(c->ev[2], c->st));
         std::vector<cth_dpres> res2(nc);
         CTH_HIP_CHECK(hipMemcpyAsync(res2.data(), d_res2, sizeof(cth_dpres) * nc, hipMemcpyDeviceToHost, c->st));
@@ -409,20 +463,25 @@ int search_run(cth_ctx* c, int q0, int q1, const cth_search_opts* opt, cth_hit* 
       // MAP stage for the passed profiles within the E-value threshold (the
       // reference's MAP kernels return early for the others,
       // cumapdp/CuBatchMAPDP_fwd.cu:127-130)
-      std::vector<int> sel;
+      // (jobs within the LDS row limit first, then the longer ones)
+      std::vector<int> sel, sel_long;
       for (int j = 0; j < np2; j++)
-        if (!(logethld < (float)sv[(size_t)j * CTH_STATS_NOUT + CTH_ST_LOGE])) sel.push_back(j);
+        if (!(logethld < (float)sv[(size_t)j * CTH_STATS_NOUT + CTH_ST_LOGE]))
+          (db.h_len[abs_[j]] > CTH_LDS_ROW_MAX ? sel_long : sel).push_back(j);
+      const int nms = (int)sel.size();
+      sel.insert(sel.end(), sel_long.begin(), sel_long.end());
       std::vector<cth_mapres> mres(sel.size());
       std::vector<char> hrows;  // alignment rows of the jobs, in traceback order
       int rowcap = 0;
       if (opt->do_map && !sel.empty()) {
-        const int nj = (int)sel.size();
+        const int nj = (int)sel.size(), nml = nj - nms;
         std::vector<cth_mapjob> jobs(nj);
-        int ltm = 0;
+        int ltm = 0, ltms = 0;
         for (int k = 0; k < nj; k++) {
           jobs[k].profile = abs_[sel[k]];
           jobs[k].logepa = (float)sv[(size_t)sel[k] * CTH_STATS_NOUT + CTH_ST_LOGEPA];
           ltm = std::max(ltm, db.h_len[jobs[k].profile]);
+          if (k < nms) ltms = ltm;
         }
         float* const post = bufs.map_buffer(lq, ldsc);
         if (!post) return -1;
@@ -430,16 +489,30 @@ int search_run(cth_ctx* c, int q0, int q1, const cth_search_opts* opt, cth_hit* 
         const size_t alnbytes = opt->alnbuf ? (size_t)nj * 5 * rowcap : 0;
         const size_t wneed = sizeof(double) * (size_t)nj * lq + sizeof(cth_mapjob) * nj + sizeof(cth_mapres) * nj +
                              alnbytes + 512;
-        double* d_zrow = (double*)bufs.work(wneed);
+        int nbm = nml;
+        if (nml) {
+          nbm = bufs.row_batch(cth_map_row_bytes(1, ltm), wneed, nml);
+          if (nbm < 1) return -1;
+        }
+        double* d_zrow = (double*)bufs.work(wneed + (nml ? cth_map_row_bytes(nbm, ltm) + 256 : 0));
         if (!d_zrow) return -1;
         cth_mapjob* d_jobs = (cth_mapjob*)(d_zrow + (size_t)nj * lq);
         cth_mapres* d_res = (cth_mapres*)(((uintptr_t)(d_jobs + nj) + 15) & ~(uintptr_t)15);
         char* d_aln = alnbytes ? (char*)(d_res + nj) : nullptr;
+        float* grow = nml ? (float*)(((uintptr_t)((char*)(d_res + nj) + alnbytes) + 255) & ~(uintptr_t)255) : nullptr;
         CTH_HIP_CHECK(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(cth_mapjob) * nj, hipMemcpyHostToDevice, c->st));
         if (timing) CTH_HIP_CHECK(hipEventRecord(c->ev[3], c->st));
-        if (cth_launch_map(&c->q.d, qi, lq, &db.d, d_jobs, nj, ltm, col0, ldsc, scores, mods, post, btck, d_zrow,
-                           opt->minpp > 0.0f ? opt->minpp : CTH_MINPP, d_aln, d_res, c->st))
+        const float minpp = opt->minpp > 0.0f ? opt->minpp : CTH_MINPP;
+        if (cth_launch_map(&c->q.d, qi, lq, &db.d, d_jobs, nms, ltms, col0, ldsc, scores, mods, post, btck, d_zrow,
+                           minpp, d_aln, rowcap, d_res, nullptr, c->st))
           return -1;
+        for (int j = nms; j < nj; j += nbm) {
+          const int n = std::min(nbm, nj - j);
+          if (cth_launch_map(&c->q.d, qi, lq, &db.d, d_jobs + j, n, ltm, col0, ldsc, scores, mods, post, btck,
+                             d_zrow + (size_t)j * lq, minpp, d_aln ? d_aln + (size_t)j * 5 * rowcap : nullptr, rowcap,
+                             d_res + j, grow, c->st))
+            return -1;
+        }
         if (timing) CTH_HIP_CHECK(hipEventRecord(c->ev[0], c->st));
         CTH_HIP_CHECK(hipMemcpyAsync(mres.data(), d_res, sizeof(cth_mapres) * nj, hipMemcpyDeviceToHost, c->st));
         if (alnbytes) {

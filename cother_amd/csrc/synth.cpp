@@ -330,7 +330,11 @@ int cth_pset_set_text(cth_pset* ps, int k, const char* name, const char* desc) {
 cth_pset* cth_pset_alloc(int nprof, const int32_t* lens) {
   if (nprof < 0 || (nprof > 0 && !lens)) { cth_set_error("pset_alloc: %d profiles", nprof); return nullptr; }
   for (int p = 0; p < nprof; p++)
-    if (lens[p] < 1) { cth_set_error("pset_alloc: profile %d has length %d", p, (int)lens[p]); return nullptr; }
+    if (lens[p] < 1 || lens[p] > CTH_MAXCOLUMNS) {
+      // the readers accept 1..MAXCOLUMNS positions (liblib/mylimits.h:50; IOProfileModelBin.cu:166)
+      cth_set_error("pset_alloc: profile %d has length %d (1..%d)", p, (int)lens[p], CTH_MAXCOLUMNS);
+      return nullptr;
+    }
   cth_pset* ps = (cth_pset*)calloc(1, sizeof(cth_pset));
   if (!ps) return nullptr;
   ps->nprof = nprof;

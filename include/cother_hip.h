@@ -73,6 +73,24 @@ cth_pset* cth_read_binary_db(const char* path);
  * values are scaled integers (scale 10000 in the reference's files). */
 int cth_write_text_profiles(const cth_pset* ps, const char* path, int scale);
 
+/* Profile sets from the reference's packed profile-model buffers: pmbeg[k]
+ * points at the array of field k of TPM2DVectorFields (181 fields,
+ * src/libmycu/cupro/PM2DVectorFields.h:53-87) as TdDataReader::
+ * ReadDataChunkHelper packs a database chunk (src/libmycu/cupro/
+ * TdDataReader.cpp:703-980) and PMBatchProData the query batch.  nprof < 0:
+ * every profile, counted from pmend (each field's end); nprof >= 0: the
+ * first nprof profiles from pmbeg (pmend may be NULL), as a worker reads one
+ * query once PMDataSkipNPros has advanced the pointers (src/libmycu/cualn/
+ * DevCommThread.cpp:314-320).  szfields (optional): each field's byte offset
+ * from pmbeg[0] (szCpm2dvfields), checked against pmbeg.  descs (optional):
+ * one description per profile.  Field sizes, lengths (1..50000, MAXCOLUMNS)
+ * and the position offsets are checked; NULL on error. */
+cth_pset* cth_pset_from_packed(char* const* pmbeg, char* const* pmend, const size_t* szfields, int nprof,
+                               const char* const* descs);
+/* Number of profiles and positions of a set. */
+int cth_pset_nprof(const cth_pset* ps);
+int64_t cth_pset_npos(const cth_pset* ps);
+
 /* Distance files (adddist: ReadDistances, GetPositionPairHelper,
  * GetPairDistanceHelper and ProcessDistancesForPosition, src/adddist/
  * adddist.cpp:267-651): lines "i j d.. [p..]" of residue pairs (1-based,
@@ -113,6 +131,14 @@ int cth_upload_queries(cth_ctx* c, const cth_pset* qs);
  * CuBatchProcessing::TransferCPMData, src/libmycu/cupro/CuBatchProcessing.cuh:
  * 60-70, with a one-time upload). */
 int cth_upload_db(cth_ctx* c, const cth_pset* db);
+
+/* The two uploads from packed buffers (cth_pset_from_packed): the database
+ * chunk as CuBatchProcessing::TransferCPMData receives it
+ * (src/libmycu/cupro/CuBatchProcessing.cuh:62-70, CuDeviceMemory::
+ * CopyCPMDataToDevice, src/libmycu/cupro/CuDeviceMemory.cu:1095-1157), and
+ * the query batch (or its first nprof queries from pmbeg). */
+int cth_upload_db_packed(cth_ctx* c, char* const* pmbeg, char* const* pmend, const size_t* szfields);
+int cth_upload_queries_packed(cth_ctx* c, char* const* pmbeg, char* const* pmend, const size_t* szfields, int nprof);
 
 /* Score matrix of query qidx against template profiles [p0,p1), copied to
  * host row-major [Lq][ncols] (ncols = positions of those profiles).

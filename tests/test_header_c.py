@@ -61,3 +61,28 @@ def test_header_compiles_as_c99(tmp_path):
 @pytest.mark.skipif(shutil.which("g++") is None, reason="no C++ compiler")
 def test_header_compiles_as_cpp17(tmp_path):
     _compile("g++", "c++17", tmp_path, "cpp")
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None or
+                    not os.path.exists(os.path.join(ROOT, "cother_amd", "libcother_hip.so")),
+                    reason="no C compiler or library not built")
+def test_c_program_links_and_starts(tmp_path):
+    """the C99 program links against libcother_hip.so with every symbol it
+    calls resolved, and starts (without a device it only counts devices)"""
+    src = tmp_path / "prog.c"
+    src.write_text(PROGRAM + r"""
+int main(int argc, char** argv) {
+  int n = cth_device_count();
+  printf("devices %d\n", n);
+  if (argc > 1 && n > 0) return run(0, argv[1], 0, 0, 0, 0);
+  return 0;
+}
+""")
+    exe = str(tmp_path / "prog")
+    libdir = os.path.join(ROOT, "cother_amd")
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", f"-I{ROOT}/include", str(src), f"-L{libdir}",
+                        "-lcother_hip", f"-Wl,-rpath,{libdir}", "-o", exe], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    run = subprocess.run([exe], capture_output=True, text=True, timeout=120,
+                         env={**os.environ, "HIP_VISIBLE_DEVICES": ""})
+    assert run.returncode == 0 and "devices" in run.stdout, run.stdout + run.stderr

@@ -103,3 +103,18 @@ def test_pset_alloc_refuses_empty_profiles(lens):
     p = lib().cth_pset_alloc(len(a), a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
     assert not p
     assert "length" in lib().cth_last_error().decode()
+
+
+def test_lengths_over_maxcolumns_refused():
+    """profiles of more than MAXCOLUMNS = 50000 positions are refused, as the
+    reference's readers refuse them (liblib/mylimits.h:50,
+    libmycu/cupro/IOProfileModelBin.cu:166)"""
+    import ctypes
+    lens = np.array([50001], np.int32)
+    p = cother_amd.lib().cth_pset_alloc(1, lens.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    assert not p
+    assert b"50000" in cother_amd.lib().cth_last_error()
+    lens = np.array([50000], np.int32)
+    p = cother_amd.lib().cth_pset_alloc(1, lens.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    assert p
+    cother_amd.lib().cth_pset_free(p)
