@@ -105,23 +105,30 @@ def cpu_baseline(q, db, target_s, threads):
             break
         nblk = int(nblk * min(8.0, max(1.5, 1.2 * target_s / max(dt, 1e-3))))
     return {"value": cells / dt / 1e9, "unit": "GCUPS", "cores": threads, "kind": "port",
+            "host_cpus": os.cpu_count(),
             "sample": f"query 0 (Lq={lq}) vs templates [{p0},{p1}) ({int(lens[p0:p1].sum())} positions), "
-                      f"{cells:.3g} cells in {dt:.1f} s, full pipeline"}
+                      f"{cells:.3g} cells in {dt:.1f} s, full pipeline, {threads} OpenMP threads "
+                      f"(the process's share of the host's {os.cpu_count()} CPUs)"}
 
 
-def serve_phase(ctx, qs, db, args, seq_qps):
+def serve_phase(ctx, qs, db, args, seq_qps, rate=None, n=None, contraction=None, first=None):
     """Requests arriving as a seeded Poisson process at serve_load x the sequential query
-    rate, run by the serving loop (slots overlap on the device, admission by buffer
-    footprint); latency is submit -> done, throughput is cells over first submit -> last done."""
+    rate (or `rate` per second), run by the serving loop (slots overlap on the device,
+    admission by buffer footprint); latency is submit -> done, throughput is cells over
+    first submit -> last done."""
     import threading
     import cother_amd
-    n = args.serve_requests
+    n = args.serve_requests if n is None else n
     ql = qs.lens()
-    order = [(args.warmup + s) % qs.nprof for s in range(n)]
-    rate = args.serve_rate if args.serve_rate > 0 else seq_qps * args.serve_load
+    first = args.warmup if first is None else first
+    order = [(first + s) % qs.nprof for s in range(n)]
+    if rate is None:
+        rate = args.serve_rate if args.serve_rate > 0 else seq_qps * args.serve_load
+    contraction = args.contraction if contraction is None else contraction
     rng = np.random.default_rng(args.seed + 7)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n))
-    srv = cother_amd.Server(ctx, nslots=args.serve_slots, mem_budget=64 << 30, chunk_cols=4 << 20, eval=args.eval)
+    srv = cother_amd.Server(ctx, nslots=args.serve_slots, mem_budget=64 << 30, chunk_cols=4 << 20, eval=args.eval,
+                            contraction=contraction)
     tickets = [None] * n
     ready = [threading.Event() for _ in range(n)]
 
@@ -153,6 +160,7 @@ def serve_phase(ctx, qs, db, args, seq_qps):
     srv.close()
     lat = np.array(lat)
     return {"requests": n, "slots": args.serve_slots, "offered_qps": round(rate, 4),
+            "contraction": ("fp32", "bf16-mfma", "fp16-mfma")[contraction],
             "offered_load": round(rate / seq_qps, 3),
             "gcups": round(cells / makespan / 1e9, 4), "p50_ms": round(float(np.percentile(lat, 50)), 2),
             "p99_ms": round(float(np.percentile(lat, 99)), 2), "mean_queued_ms": round(float(np.mean(queued)), 2),
@@ -208,11 +216,21 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=0.0,
                     help="serving phase: absolute arrival rate in queries/s (overrides --serve-load)")
     ap.add_argument("--serve-slots", type=int, default=2)
+    ap.add_argument("--serve-sweep", type=int, default=6, metavar="N",
+                    help="Poisson arrivals at 20, 50 and 100 queries/s, FP32 and BF16 contraction, N requests each "
+                         "(0: skip)")
+    ap.add_argument("--map-heavy", type=int, default=4, metavar="N",
+                    help="N queries at E-value 1e4 (many hits realigned by MAP), reported in 'map_heavy' (0: skip)")
     ap.add_argument("--contraction", type=int, default=0, choices=(0, 1, 2),
                     help="log-odds term: 0 FP32 (the reference's arithmetic), 1 BF16 / 2 FP16 matrix-core variant")
     ap.add_argument("--long-profile", action="store_true",
                     help="long-profile stress instead: one query of 2000 positions (predicted distance map) "
                          "against 5000 templates of 1000-3000 positions")
+    ap.add_argument("--workload", nargs=8, type=float, default=None,
+                    metavar=("QMED", "QMIN", "QMAX", "TMED", "TMIN", "TMAX", "QDIST", "TDIST"),
+                    help="other query/template length distributions (lognormal median, clip range, sigma 0.35) "
+                         "and distance modes (0 predicted maps, 1 C-alpha chains, 2 none) in place of the "
+                         "benchmark's 250/40/1000, 220/30/1500, 0, 1")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel over BYTES (traffic-counter calibration) and exit")
     args = ap.parse_args()
@@ -231,14 +249,18 @@ def main():
         print(json.dumps({"copy_bytes": args.copy_calibration, "launches": 4, "ms_per_launch": ms.value,
                           "GBps": 2 * args.copy_calibration / (ms.value * 1e-3) / 1e9}), flush=True)
         return
-    threads = min(16, os.cpu_count() or 1)
+    # the host threads this process may use: the OpenMP setting where the
+    # environment fixes it (the GPU host's share of cores), else the CPUs the
+    # process is allowed to run on
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
     if args.long_profile:
         long_profile_run(args, threads)
         return
     t0 = time.perf_counter()
-    qs = cother_amd.synth(args.queries, 1, 250.0, 0.35, 40, 1000, dist_mode=0, sort_desc=0, seed=args.seed,
-                          threads=threads)
-    db = cother_amd.synth(args.templates, 1, 220.0, 0.35, 30, 1500, dist_mode=1, sort_desc=1,
+    wl = args.workload or (250, 40, 1000, 220, 30, 1500, 0, 1)
+    qs = cother_amd.synth(args.queries, 1, wl[0], 0.35, int(wl[1]), int(wl[2]), dist_mode=int(wl[6]), sort_desc=0,
+                          seed=args.seed, threads=threads)
+    db = cother_amd.synth(args.templates, 1, wl[3], 0.35, int(wl[4]), int(wl[5]), dist_mode=int(wl[7]), sort_desc=1,
                           seed=args.seed + 1, threads=threads)
     log(f"workload: {qs.nprof} queries ({qs.npos} positions), {db.nprof} templates ({db.npos} positions); "
         f"generated in {time.perf_counter() - t0:.1f} s")
@@ -293,6 +315,33 @@ def main():
     serving = None
     if args.serve_requests > 0:
         serving = serve_phase(ctx, qs, db, args, args.steps / wall)
+    sweep = None
+    if args.serve_sweep > 0:
+        sweep = []
+        for prec in (0, 1):
+            for rate in (20.0, 50.0, 100.0):
+                r = serve_phase(ctx, qs, db, args, args.steps / wall, rate=rate, n=args.serve_sweep,
+                                contraction=prec, first=args.warmup + args.steps)
+                log(f"sweep: {r}")
+                sweep.append(r)
+    map_heavy = None
+    if args.map_heavy > 0:
+        mh_cells, mh_map_ms, mh_map_cells, mh_hits = 0.0, 0.0, 0.0, 0
+        t1 = time.perf_counter()
+        for s in range(args.map_heavy):
+            qi = (args.warmup + s) % qs.nprof
+            hits, nh, st = ctx.search(qi, qi + 1, eval=1e4, timing=True, contraction=args.contraction)
+            mh_cells += st.cells
+            mh_map_ms += st.ms_k_map
+            mh_map_cells += st.map_cells
+            mh_hits += int(nh[0])
+            log(f"map-heavy {s}: query {qi}, {int(nh[0])} hits, map {st.ms_k_map:.0f} ms")
+        mh_wall = time.perf_counter() - t1
+        map_heavy = {"queries": args.map_heavy, "eval": 1e4, "gcups": round(mh_cells / mh_wall / 1e9, 4),
+                     "hits_per_query": round(mh_hits / args.map_heavy, 1), "map_ms": round(mh_map_ms, 1),
+                     "map_cells": mh_map_cells,
+                     "map_gcups": round(mh_map_cells / (mh_map_ms * 1e-3) / 1e9, 4) if mh_map_ms > 0 else None,
+                     "ms_per_query": round(mh_wall * 1e3 / args.map_heavy, 1)}
     cpu = None
     if args.cpu_seconds > 0:
         log("cpu baseline ...")
@@ -302,8 +351,9 @@ def main():
         "metric": METRIC, "value": round(gcups, 4), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 2), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "64 queries (Lq lognormal median 250, 40-1000) vs resident DB of 60000 templates "
-                               "(lognormal median 220, 30-1500); one query per step",
+        "config": {"workload": (f"{qs.nprof} queries (Lq lognormal median {wl[0]:g}, {wl[1]:g}-{wl[2]:g}) vs resident "
+                                f"DB of {db.nprof} templates (lognormal median {wl[3]:g}, {wl[4]:g}-{wl[5]:g}); one "
+                                f"query per step"),
                    "queries": qs.nprof, "templates": db.nprof, "db_positions": int(db.npos),
                    "query_positions_timed": int(sum(ql[(args.warmup + s) % qs.nprof] for s in range(args.steps))),
                    "eval": args.eval, "map": True,
@@ -313,7 +363,8 @@ def main():
                      "map": round(ms_map, 1)},
         "passed_per_query": round(npass / args.steps, 1), "hits_per_query": round(nhits / args.steps, 1),
         "map_cells": map_cells,
-        "roofline": roofline, "cpu_baseline": cpu, "serving": serving,
+        "roofline": roofline, "cpu_baseline": cpu, "serving": serving, "serving_sweep": sweep,
+        "map_heavy": map_heavy,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -120,6 +120,15 @@ def lib():
     L.cth_search.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(SearchOpts),
                              ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(SearchStats)]
     L.cth_search_noalloc.argtypes = L.cth_search.argtypes
+    VP = ctypes.POINTER(ctypes.c_void_p)
+    L.cth_pset_from_packed.restype = P
+    L.cth_pset_from_packed.argtypes = [VP, VP, ctypes.POINTER(ctypes.c_size_t), ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_char_p)]
+    L.cth_pset_nprof.argtypes = [P]
+    L.cth_pset_npos.restype = ctypes.c_int64
+    L.cth_pset_npos.argtypes = [P]
+    L.cth_upload_db_packed.argtypes = [ctypes.c_void_p, VP, VP, ctypes.POINTER(ctypes.c_size_t)]
+    L.cth_upload_queries_packed.argtypes = [ctypes.c_void_p, VP, VP, ctypes.POINTER(ctypes.c_size_t), ctypes.c_int]
     L.cth_pset_name.restype = ctypes.c_char_p
     L.cth_pset_name.argtypes = [P, ctypes.c_int]
     L.cth_pset_desc.restype = ctypes.c_char_p
@@ -534,11 +543,12 @@ class Server:
     """The serving loop (cth_server_*) over a context's uploaded queries and resident database."""
 
     def __init__(self, ctx, nslots=2, mem_budget=64 << 30, chunk_cols=4 << 20, eval=10.0, ssemodel=2, do_map=True,
-                 minpp=MINPP, alncap=0):
+                 minpp=MINPP, alncap=0, contraction=0):
         o = ServerOpts()
         lib().cth_server_opts_init(ctypes.byref(o))
         o.nslots, o.mem_budget, o.chunk_cols, o.alncap = nslots, mem_budget, chunk_cols, alncap
         o.search.eval, o.search.ssemodel, o.search.do_map, o.search.minpp = eval, ssemodel, int(do_map), minpp
+        o.search.contraction = contraction
         self.ctx = ctx
         self.ndb = ctx.db.nprof
         self.alncap = alncap
