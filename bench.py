@@ -167,6 +167,47 @@ def serve_phase(ctx, qs, db, args, seq_qps, rate=None, n=None, contraction=None,
             "mem_budget_gib": 64, "chunk_cols": 4 << 20}
 
 
+def batch_phase(args, threads, tdist):
+    """Many short queries against a small resident database, searched one
+    query at a time (cth_search) and as one batch (cth_search_batch: one
+    launch per stage over all pairs, device-side selection, graph replay):
+    32 queries (Lq lognormal median 90, clipped 40-150, predicted maps)
+    against 4000 templates (lognormal median 220, clipped 30-400; C-alpha chain
+    distance lists, or none with tdist 2).  Both paths' hits are compared."""
+    import cother_amd
+    q = cother_amd.synth(32, 1, 90.0, 0.35, 40, 150, dist_mode=0, sort_desc=0, seed=args.seed + 21, threads=threads)
+    db = cother_amd.synth(4000, 1, 220.0, 0.35, 30, 400, dist_mode=tdist, sort_desc=1, seed=args.seed + 22,
+                          threads=threads)
+    ctx = cother_amd.Context(0)
+    ctx.upload_queries(q)
+    ctx.upload_db(db)
+    cells = float(q.lens().sum()) * db.npos
+    ctx.search(0, 2, eval=args.eval)                    # warmup
+    ctx.search(eval=args.eval, batch=True)              # autotune + graph capture
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for qi in range(q.nprof):
+            a, na, _ = ctx.search(qi, qi + 1, eval=args.eval)
+    per_query = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b, nb, sb = ctx.search(eval=args.eval, batch=True)
+    batched = time.perf_counter() - t0
+    ref, nref, _ = ctx.search(eval=args.eval)
+    same = len(ref) == len(b) and all((ref[f] == b[f]).all() for f in ("query", "profile", "score", "logE", "alnlen"))
+    caps, reps_g = ctx.graph_counts()
+    ctx.close()
+    g1, g2 = cells * reps / per_query / 1e9, cells * reps / batched / 1e9
+    log(f"batch phase (template distances {'none' if tdist == 2 else 'C-alpha'}): per-query {g1:.3f} GCUPS, "
+        f"batched {g2:.3f} GCUPS ({g2 / g1:.2f}x), hits equal: {same}")
+    return {"workload": "32 queries (Lq lognormal median 90, 40-150) vs 4000 templates (lognormal median 220, "
+                        f"30-400), template distance lists: {'none' if tdist == 2 else 'C-alpha chains'}",
+            "cells": cells, "per_query_gcups": round(g1, 4), "batched_gcups": round(g2, 4),
+            "speedup": round(g2 / g1, 3), "hits": int(len(b)), "hits_equal": bool(same),
+            "batches_per_search": sb.nchunks, "graph_captures": caps, "graph_replays": reps_g}
+
+
 def long_profile_run(args, threads):
     """One query of 2000 positions with a full predicted distance map against 5000
     templates of 1000-3000 positions (C-alpha chain distances): long DP/MAP rows, a
@@ -219,6 +260,9 @@ def main():
     ap.add_argument("--serve-sweep", type=int, default=6, metavar="N",
                     help="Poisson arrivals at 20, 50 and 100 queries/s, FP32 and BF16 contraction, N requests each "
                          "(0: skip)")
+    ap.add_argument("--batch-phase", type=int, default=1, choices=(0, 1),
+                    help="per-query vs batched search of 32 short queries against 4000 templates, with and "
+                         "without template distance lists, reported in 'batched' (0: skip)")
     ap.add_argument("--map-heavy", type=int, default=4, metavar="N",
                     help="N queries at E-value 1e4 (many hits realigned by MAP), reported in 'map_heavy' (0: skip)")
     ap.add_argument("--contraction", type=int, default=0, choices=(0, 1, 2),
@@ -324,6 +368,9 @@ def main():
                                 contraction=prec, first=args.warmup + args.steps)
                 log(f"sweep: {r}")
                 sweep.append(r)
+    batched = None
+    if args.batch_phase:
+        batched = [batch_phase(args, threads, 1), batch_phase(args, threads, 2)]
     map_heavy = None
     if args.map_heavy > 0:
         mh_cells, mh_map_ms, mh_map_cells, mh_hits = 0.0, 0.0, 0.0, 0
@@ -364,7 +411,7 @@ def main():
         "passed_per_query": round(npass / args.steps, 1), "hits_per_query": round(nhits / args.steps, 1),
         "map_cells": map_cells,
         "roofline": roofline, "cpu_baseline": cpu, "serving": serving, "serving_sweep": sweep,
-        "map_heavy": map_heavy,
+        "map_heavy": map_heavy, "batched": batched,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -120,6 +120,9 @@ def lib():
     L.cth_search.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(SearchOpts),
                              ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(SearchStats)]
     L.cth_search_noalloc.argtypes = L.cth_search.argtypes
+    L.cth_search_batch.argtypes = L.cth_search.argtypes
+    L.cth_batch_graph_counts.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64),
+                                         ctypes.POINTER(ctypes.c_int64)]
     VP = ctypes.POINTER(ctypes.c_void_p)
     L.cth_pset_from_packed.restype = P
     L.cth_pset_from_packed.argtypes = [VP, VP, ctypes.POINTER(ctypes.c_size_t), ctypes.c_int,
@@ -425,6 +428,12 @@ class Context:
         _check(L.cth_ctx_create(device, envdir.encode(), ctypes.byref(h)), "cth_ctx_create")
         self.h = h
 
+    def graph_counts(self):
+        """(captures, replays) of the batched search's graph on this context."""
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().cth_batch_graph_counts(self.h, ctypes.byref(a), ctypes.byref(b)), "cth_batch_graph_counts")
+        return a.value, b.value
+
     def close(self):
         if getattr(self, "h", None) and self.h.value:
             lib().cth_ctx_destroy(self.h)
@@ -489,12 +498,14 @@ class Context:
         return res
 
     def search(self, q0=0, q1=None, eval=10.0, ssemodel=2, do_map=True, timing=False, mem_budget=0,
-               maxhits=None, minpp=MINPP, alignments=False, contraction=0, dbsize=0, ndb=0, noalloc=False):
+               maxhits=None, minpp=MINPP, alignments=False, contraction=0, dbsize=0, ndb=0, noalloc=False,
+               batch=False):
         """Search queries [q0,q1) against the resident database.
 
         dbsize, ndb: positions and profiles of the whole database when the resident set is one
         part of it (0, 0: the resident set's own).  noalloc: run cth_search_noalloc, whose
         buffers come from the library's fixed per-device scratch array (mem_budget unused).
+        batch: run cth_search_batch (many queries per launch, graph-replayed).
 
         Returns (hits HIT array grouped by query, per-query hit counts, SearchStats), and with
         alignments=True also a list of five-row alignments (query, middle, target, query SS,
@@ -513,9 +524,10 @@ class Context:
         opts = SearchOpts(eval, ssemodel, int(do_map), int(timing), mem_budget, minpp, contraction,
                           ctypes.cast(abuf, ctypes.c_void_p) if abuf is not None else None,
                           len(abuf) if abuf is not None else 0, dbsize, ndb)
-        fn = lib().cth_search_noalloc if noalloc else lib().cth_search
+        name = "cth_search_noalloc" if noalloc else ("cth_search_batch" if batch else "cth_search")
+        fn = getattr(lib(), name)
         _check(fn(self.h, q0, q1, ctypes.byref(opts), hits.ctypes.data, maxhits, nh.ctypes.data, ctypes.byref(st)),
-               "cth_search_noalloc" if noalloc else "cth_search")
+               name)
         hits = hits[:int(nh.sum())]
         if not alignments:
             return hits, nh, st

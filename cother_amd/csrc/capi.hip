@@ -131,7 +131,11 @@ extern "C" void cth_ctx_destroy(cth_ctx* c) {
     if (p) (void)hipFree(p);
   if (c->h_pin) (void)hipHostFree(c->h_pin);
   if (c->has_ev)
-    for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
+    for (int k = 0; k < 4; k++) (void)hipEventDestroy(c->ev[k]);
+  if (c->nbev == 5) (void)hipEventDestroy(c->ev[4]);
+  if (c->has_bgraph) (void)hipGraphExecDestroy(c->bgraph);
+  if (c->bcells) (void)hipFree(c->bcells);
+  if (c->bsmall) (void)hipFree(c->bsmall);
   if (c->st) (void)hipStreamDestroy(c->st);
   free(c);
 }

@@ -9,6 +9,13 @@
 #include "cth_devdb.h"
 #include "cth_tables.h"
 
+// an autotuned launch shape of the batched search, per batch shape
+struct cth_btune {
+  uint64_t key;
+  int dp_grid, small_grid;
+  float ms;
+};
+
 struct cth_ctx {
   int device;
   hipStream_t st;
@@ -25,8 +32,16 @@ struct cth_ctx {
   float* mapbuf; size_t cap_mapbuf;  // MAP posteriors [Lq][ldsc]
   float* grow; size_t cap_grow;      // DP/MAP stripe boundary rows of profiles over the LDS row limit
   void* h_pin; size_t cap_pin;
-  // stage events of cth_search's timing mode (created on first use)
-  hipEvent_t ev[4]; int has_ev;
+  // stage events of cth_search's timing mode (created on first use; the
+  // batched search uses a fifth)
+  hipEvent_t ev[5]; int has_ev; int nbev;
+  // batched search (batch.hip): cell buffers and tables of a batch, the
+  // captured graph of the last batch shape and the tuned grid sizes
+  void* bcells; size_t cap_bcells;
+  void* bsmall; size_t cap_bsmall;
+  hipGraphExec_t bgraph; int has_bgraph; uint64_t bkey[12];
+  int64_t nbgraph_captures, nbgraph_replays;
+  cth_btune btune[32]; int nbtune;
   // a context made by cth_ctx_create_shared reads its parent's tables,
   // queries and database and frees only its own stream and scratch
   int borrowed;

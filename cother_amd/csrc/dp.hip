@@ -27,6 +27,7 @@
 // and begin coordinates.
 #include <hip/hip_runtime.h>
 
+#include "cth_batch.h"
 #include "cth_dev.h"
 #include "cth_dp.h"
 
@@ -51,20 +52,15 @@ __device__ __forceinline__ bool in_valid_area(int lq, int lt, int edge, int y, i
   return !corners && 0 <= y && y < lq && 0 <= x && x < lt;
 }
 
-template <bool kGlobalRow>
-__global__ __launch_bounds__(64) void k_dp(cth_dpset q, int qp, cth_dpset db, int pbase,
-                                           const int* __restrict__ plist, int64_t col0, int64_t ldsc,
-                                           const float* __restrict__ scores, uint8_t* __restrict__ btck,
-                                           cth_dpres* __restrict__ res, float* __restrict__ grow,
-                                           int64_t growstride) {
-  // [5][lt]: bottom row of the previous stripe, in LDS or in this
-  // wavefront's growstride floats of grow
-  extern __shared__ float lds_row[];
-  float* const bot = kGlobalRow ? grow + (int64_t)blockIdx.x * growstride : lds_row;
+// The DP of query qp against template profile p, whose scores begin at
+// scores[colp] (row stride ldsc); traceback bytes to btck at the same offsets
+// (NULL: none); bot: the stripe boundary rows [5][lt] (LDS or global).  The
+// first maximum in row-major order and its coordinates reach every lane.
+__device__ __forceinline__ void dp_pair(const cth_dpset& q, int qp, const cth_dpset& db, int p, int64_t colp,
+                                        int64_t ldsc, const float* __restrict__ scores, uint8_t* __restrict__ btck,
+                                        float* bot, float& best_out, uint32_t& bestxy_out) {
   const int lane = threadIdx.x;
-  const int p = plist ? plist[blockIdx.x] : pbase + blockIdx.x;
   const int lq = q.len[qp], lt = db.len[p];
-  const int64_t colp = db.dist[p] - col0;
   const int64_t qb = q.dist[qp] + qp;      // query transition vector 0
   const int64_t tb = db.dist[p] + p;       // template transition vector 0
   const int edge = corner_maxlen(lq, lt);
@@ -210,25 +206,71 @@ __global__ __launch_bounds__(64) void k_dp(cth_dpset q, int qp, cth_dpset db, in
     const bool take = ob > best || (ob == best && (oy < my || (oy == my && (oxy >> 16) < (bestxy >> 16))));
     if (take) { best = ob; bestxy = oxy; }
   }
-  if (lane == 0) {
+  best_out = best;
+  bestxy_out = bestxy;
+}
+
+template <bool kGlobalRow>
+__global__ __launch_bounds__(64) void k_dp(cth_dpset q, int qp, cth_dpset db, int pbase,
+                                           const int* __restrict__ plist, int64_t col0, int64_t ldsc,
+                                           const float* __restrict__ scores, uint8_t* __restrict__ btck,
+                                           cth_dpres* __restrict__ res, float* __restrict__ grow,
+                                           int64_t growstride) {
+  // [5][lt]: bottom row of the previous stripe, in LDS or in this
+  // wavefront's growstride floats of grow
+  extern __shared__ float lds_row[];
+  float* const bot = kGlobalRow ? grow + (int64_t)blockIdx.x * growstride : lds_row;
+  const int p = plist ? plist[blockIdx.x] : pbase + blockIdx.x;
+  float best;
+  uint32_t bestxy;
+  dp_pair(q, qp, db, p, db.dist[p] - col0, ldsc, scores, btck, bot, best, bestxy);
+  if (threadIdx.x == 0) {
     res[blockIdx.x].maxsc = best;
     res[blockIdx.x].maxxy = bestxy;
   }
 }
 
-// lane 0 of each wavefront finalises one profile
-__global__ __launch_bounds__(64) void k_dpfin(cth_dpset db, int pbase, const int* __restrict__ plist, int64_t col0,
-                                              int64_t ldsc, const float* __restrict__ scores,
-                                              const uint8_t* __restrict__ btck, float scorethld,
-                                              cth_dpres* __restrict__ res) {
-  __shared__ float cache[36];
-  __shared__ float tree[32];
-  if (threadIdx.x != 0) return;
-  const int p = plist ? plist[blockIdx.x] : pbase + blockIdx.x;
-  const int64_t colp = db.dist[p] - col0;
-  cth_dpres r = res[blockIdx.x];
+// Batched phase 1: a persistent grid works through n pairs (or the *count
+// pairs listed in list), each wavefront taking the next pair from the device
+// counter next; pair pairs[i] of query bq[pairs[i].b] reads that query's
+// scores (and writes its traceback bytes) at the query's offset; res is per
+// pair.  Pairs come in the order given (longest first), so the longest DPs
+// start first.
+template <bool kTrace>
+__global__ __launch_bounds__(64) void k_dp_batch(cth_dpset q, cth_dpset db, const cth_bquery* __restrict__ bq,
+                                                 const cth_bpair* __restrict__ pairs, const int* __restrict__ list,
+                                                 const int* __restrict__ count, int n, int* __restrict__ next,
+                                                 int64_t ldsc, const float* __restrict__ scores,
+                                                 uint8_t* __restrict__ btck, cth_dpres* __restrict__ res) {
+  extern __shared__ float lds_row[];
+  const int nj = count ? *count : n;
+  for (;;) {
+    int k = 0;
+    if (threadIdx.x == 0) k = atomicAdd(next, 1);
+    k = __builtin_amdgcn_readfirstlane(__shfl(k, 0));  // uniform: scalar loads of the pair and query
+    if (k >= nj) break;
+    const int i = list ? list[k] : k;
+    const cth_bpair pr = pairs[i];
+    const cth_bquery Q = bq[pr.b];
+    float best;
+    uint32_t bestxy;
+    dp_pair(q, Q.qp, db, pr.p, db.dist[pr.p], ldsc, scores + Q.off, kTrace ? btck + Q.off : nullptr, lds_row, best,
+            bestxy);
+    if (threadIdx.x == 0) {
+      res[i].maxsc = best;
+      res[i].maxxy = bestxy;
+    }
+  }
+}
+
+// FinalizeDPPro for one profile (lane 0): pass test against scorethld,
+// traceback from the maximum, correlation bonus, signed square root,
+// positive-score count and begin coordinates into r.
+__device__ __forceinline__ void dpfin_pair(const cth_dpset& db, int p, int64_t colp, int64_t ldsc, const float* __restrict__ scores,
+                           const uint8_t* __restrict__ btck, float scorethld, cth_dpres& r, float* cache,
+                           float* tree) {
   r.passed = 0; r.score = 0.0f; r.psts = 0; r.beg = 0;
-  if (r.maxsc <= scorethld) { res[blockIdx.x] = r; return; }
+  if (r.maxsc <= scorethld) return;
   int x = (int)(r.maxxy >> 16), y = (int)(r.maxxy & 0xffff);
   int psts = 0;
   for (int k = 0; k < 36; k++) cache[k] = 0.0f;
@@ -268,7 +310,41 @@ __global__ __launch_bounds__(64) void k_dpfin(cth_dpset db, int pbase, const int
   r.beg = ((uint32_t)(x + 1) << 16) | (uint32_t)(y + 1);
   r.score = score;
   r.passed = scorethld < score ? 1 : 0;
+}
+
+// lane 0 of each wavefront finalises one profile
+__global__ __launch_bounds__(64) void k_dpfin(cth_dpset db, int pbase, const int* __restrict__ plist, int64_t col0,
+                                              int64_t ldsc, const float* __restrict__ scores,
+                                              const uint8_t* __restrict__ btck, float scorethld,
+                                              cth_dpres* __restrict__ res) {
+  __shared__ float cache[36];
+  __shared__ float tree[32];
+  if (threadIdx.x != 0) return;
+  const int p = plist ? plist[blockIdx.x] : pbase + blockIdx.x;
+  cth_dpres r = res[blockIdx.x];
+  dpfin_pair(db, p, db.dist[p] - col0, ldsc, scores, btck, scorethld, r, cache, tree);
   res[blockIdx.x] = r;
+}
+
+// Batched finalisation over the *count pairs listed in list (persistent,
+// lane 0 of each wavefront), each against its query's threshold.
+__global__ __launch_bounds__(64) void k_dpfin_batch(cth_dpset db, const cth_bquery* __restrict__ bq,
+                                                    const cth_bpair* __restrict__ pairs, const int* __restrict__ list,
+                                                    const int* __restrict__ count, int64_t ldsc,
+                                                    const float* __restrict__ scores,
+                                                    const uint8_t* __restrict__ btck, cth_dpres* __restrict__ res) {
+  __shared__ float cache[36];
+  __shared__ float tree[32];
+  if (threadIdx.x != 0) return;
+  const int nj = *count;
+  for (int k = blockIdx.x; k < nj; k += gridDim.x) {
+    const int i = list[k];
+    const cth_bpair pr = pairs[i];
+    const cth_bquery Q = bq[pr.b];
+    cth_dpres r = res[i];
+    dpfin_pair(db, pr.p, db.dist[pr.p], ldsc, scores + Q.off, btck + Q.off, Q.scorethld, r, cache, tree);
+    res[i] = r;
+  }
 }
 
 __global__ void k_selftest_dpp(int* out) {
@@ -329,4 +405,32 @@ extern "C" int cth_launch_dp(const cth_dpset* q, int qp, const cth_dpset* db, in
                              int64_t col0, int64_t ldsc, const float* scores, uint8_t* btck, float scorethld,
                              cth_dpres* res, float* grow, hipStream_t st) {
   return cth_launch_dp_list(q, qp, db, p0, nullptr, np, ltmax, col0, ldsc, scores, btck, scorethld, 1, res, grow, st);
+}
+
+// Batched phase 1 (k_dp_batch) on a persistent grid of `grid` wavefronts:
+// the first pass over n pairs (list NULL, btck NULL), or the traceback pass
+// over the *count pairs of list followed by their finalisation (btck not
+// NULL).  next: a device counter that starts at zero.  Every profile of the
+// database fits the LDS row limit (the batched path keeps boundary rows in
+// LDS; ltmax is the database's longest profile).
+extern "C" int cth_launch_dp_batch(const cth_dpset* q, const cth_dpset* db, const cth_bquery* bq,
+                                   const cth_bpair* pairs, const int* list, const int* count, int n, int* next,
+                                   int ltmax, int64_t ldsc, const float* scores, uint8_t* btck, cth_dpres* res,
+                                   int grid, hipStream_t st) {
+  if (ltmax > CTH_LDS_ROW_MAX) { cth_set_error("dp batch: profile length %d over the LDS row limit", ltmax); return -1; }
+  if (grid < 1) return 0;
+  const size_t lds = sizeof(float) * 5 * (size_t)(ltmax > 0 ? ltmax : 1);
+  if (btck)
+    hipLaunchKernelGGL(k_dp_batch<true>, dim3(grid), dim3(64), lds, st, *q, *db, bq, pairs, list, count, n, next, ldsc,
+                       scores, btck, res);
+  else
+    hipLaunchKernelGGL(k_dp_batch<false>, dim3(grid), dim3(64), lds, st, *q, *db, bq, pairs, list, count, n, next, ldsc,
+                       scores, btck, res);
+  CTH_HIP_CHECK(hipGetLastError());
+  if (btck && list) {
+    hipLaunchKernelGGL(k_dpfin_batch, dim3(grid), dim3(64), 0, st, *db, bq, pairs, list, count, ldsc, scores, btck,
+                       res);
+    CTH_HIP_CHECK(hipGetLastError());
+  }
+  return 0;
 }

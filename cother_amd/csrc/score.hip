@@ -22,6 +22,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "cth_batch.h"
 #include "cth_dev.h"
 #include "cth_math.h"
 
@@ -277,17 +278,29 @@ __device__ __forceinline__ float ddms_cell(const float* __restrict__ dmsT, const
 
 // kPreInit: the log-odds term is already in scores (the reduced-precision
 // contraction of init_mfma.hip); otherwise it is computed here in FP32.
-template <bool kPreInit>
+// kBatch: the row tiles of several queries in one launch, blockIdx.y indexing
+// ytab (query of the batch and first row), each query's blocks at its offset.
+template <bool kPreInit, bool kBatch = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_score(cth_dpset q, int qp, cth_dpset db, int64_t col0, int ncols,
                                                int64_t ldsc, const cth_tables* __restrict__ tabs,
                                                float* __restrict__ scores, float* __restrict__ mods,
-                                               const int32_t* __restrict__ qoff) {
+                                               const int32_t* __restrict__ qoff,
+                                               const cth_bytile* __restrict__ ytab = nullptr,
+                                               const cth_bquery* __restrict__ bq = nullptr) {
   __shared__ ScoreLds L;
   const int tid = threadIdx.x;
   const int lane = tid & 63, w = tid >> 6;
+  int row = blockIdx.y * kRows + w;
+  if constexpr (kBatch) {
+    const cth_bytile t = ytab[blockIdx.y];
+    const cth_bquery Q = bq[t.b];
+    qp = Q.qp;
+    row = t.row0 + w;
+    scores += Q.off;
+    mods += Q.off;
+  }
   const int lq = q.len[qp];
   const int64_t q0 = q.dist[qp];
-  const int row = blockIdx.y * kRows + w;
   const int col = blockIdx.x * kCols + lane;
 
   // stage tables
@@ -410,3 +423,24 @@ extern "C" int cth_launch_score(const cth_dpset* q, const int32_t* qoff, int qp,
   CTH_HIP_CHECK(hipGetLastError());
   return 0;
 }
+
+// The score stage of a batch of queries in one launch: nytiles row tiles
+// (ytab: query of the batch and first row, kRows rows each) against the
+// whole resident database (ncols positions from column 0); each query's
+// scores and modular scores at its offset (bq[].off, row stride ldsc).
+extern "C" int cth_launch_score_batch(const cth_dpset* q, const int32_t* qoff, const cth_dpset* db, int ncols,
+                                      int64_t ldsc, const cth_tables* dtabs, const cth_bytile* ytab, int nytiles,
+                                      const cth_bquery* bq, float* scores, float* mods, hipStream_t st) {
+  if (ncols <= 0 || nytiles <= 0) return 0;
+  if (!qoff) { cth_set_error("score batch: query distance offsets not built"); return -1; }
+  if (ldsc < ncols) { cth_set_error("score batch: ldsc %lld < ncols %d", (long long)ldsc, ncols); return -1; }
+  if (nytiles > 65535) { cth_set_error("score batch: %d row tiles (at most 65535)", nytiles); return -1; }
+  dim3 grid((ncols + kCols - 1) / kCols, nytiles);
+  hipLaunchKernelGGL((k_score<false, true>), grid, dim3(256), 0, st, *q, 0, *db, (int64_t)0, ncols, ldsc, dtabs,
+                     scores, mods, qoff, ytab, bq);
+  CTH_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+// rows of one tile of the score kernel
+extern "C" int cth_score_rows_per_tile(void) { return kRows; }

@@ -253,6 +253,14 @@ extern "C" void cth_search_space(uint64_t qlen, uint64_t dbsize, uint64_t ndb, u
   if (deltalen) *deltalen = dl;
 }
 
+// A query's score threshold, log E-value threshold and length-adjusted
+// search space against dbsize positions in ndb profiles.
+extern "C" void cth_query_thresholds(float eval, uint64_t lq, uint64_t dbsize, uint64_t ndb, float* scorethld,
+                                     float* logethld, double* sspace) {
+  score_threshold(eval, lq, dbsize, scorethld, logethld);
+  *sspace = (double)length_adjustment(lq, dbsize, ndb);
+}
+
 // Option checks shared by every entry point that takes cth_search_opts.
 extern "C" int cth_search_opts_check(const cth_search_opts* opt, const char* who) {
   if (!opt) { cth_set_error("%s: no options", who); return -1; }
@@ -293,7 +301,7 @@ int search_run(cth_ctx* c, int q0, int q1, const cth_search_opts* opt, cth_hit* 
   const size_t budget = opt->mem_budget ? opt->mem_budget : ((size_t)64 << 30);
   const bool timing = opt->timing != 0;
   if (timing && !c->has_ev) {
-    for (hipEvent_t& e : c->ev) CTH_HIP_CHECK(hipEventCreate(&e));
+    for (int k = 0; k < 4; k++) CTH_HIP_CHECK(hipEventCreate(&c->ev[k]));
     c->has_ev = 1;
   }
   int nh = 0;

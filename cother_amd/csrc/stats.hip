@@ -19,6 +19,7 @@
 //           6.2, 6.4), in double precision.
 #include <hip/hip_runtime.h>
 
+#include "cth_batch.h"
 #include "cth_dev.h"
 #include "cth_dp.h"
 #include "cth_stats.h"
@@ -48,14 +49,11 @@ __constant__ float c_nnw[93] = {
     0.1f, 12.0f, 1.0f, 4.0f, 0.05f, 1.0f, 0.1f, 0.35f, 0.65f,
     0.6f, 16.0f, 1.0f, 12.0f, 0.3f, -1.7f, 0.1f, 0.45f, 0.65f};
 
-__global__ __launch_bounds__(256) void k_hist(const cth_dpset q, int qp, const cth_dpset db, int p0, int64_t col0,
-                                              int64_t ldsc, const float* __restrict__ scores,
-                                              const int* __restrict__ plist, int* __restrict__ hist) {
-  __shared__ int h[kNssc];
-  const int k = blockIdx.x;
-  const int p = plist[k];
+// The 641-bin score histogram of query qp against profile p (scores from
+// scores[colp], row stride ldsc) into the workgroup's LDS array h.
+__device__ __forceinline__ void hist_fill(const cth_dpset& q, int qp, const cth_dpset& db, int p, int64_t colp,
+                                          int64_t ldsc, const float* __restrict__ scores, int* h) {
   const int lq = q.len[qp], lt = db.len[p];
-  const int64_t colp = db.dist[p] - col0;
   const char* qaa = q.aa + q.dist[qp];
   const char* taa = db.aa + db.dist[p];
   for (int b = threadIdx.x; b < kNssc; b += blockDim.x) h[b] = 0;
@@ -70,6 +68,15 @@ __global__ __launch_bounds__(256) void k_hist(const cth_dpset q, int qp, const c
     atomicAdd(&h[(int)rintf(s * 32.0f) - kSscMin], 1);
   }
   __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_hist(const cth_dpset q, int qp, const cth_dpset db, int p0, int64_t col0,
+                                              int64_t ldsc, const float* __restrict__ scores,
+                                              const int* __restrict__ plist, int* __restrict__ hist) {
+  __shared__ int h[kNssc];
+  const int k = blockIdx.x;
+  const int p = plist[k];
+  hist_fill(q, qp, db, p, db.dist[p] - col0, ldsc, scores, h);
   for (int b = threadIdx.x; b < kNssc; b += blockDim.x) hist[(int64_t)k * kNssc + b] = h[b];
 }
 
@@ -317,19 +324,16 @@ __device__ void significance(int ssemodel, int lq, int lt, double qeno, double t
   out[2] = logexp;
 }
 
-__global__ __launch_bounds__(64) void k_stats(const cth_dpset q, int qp, const cth_dpset db,
-                                              const int* __restrict__ plist, const int* __restrict__ prel,
-                                              const cth_dpres* __restrict__ dpres,
-                                              const int* __restrict__ hist, double searchspace, int ssemodel,
-                                              double* __restrict__ out) {
+// The statistics of one passed profile p (one wavefront): parameters from
+// its histogram hc, then the significance of its phase-1 record r, into the
+// CTH_STATS_NOUT doubles at o.
+__device__ __forceinline__ void stats_one(const cth_dpset& q, int qp, const cth_dpset& db, int p, const int* hc,
+                                          double searchspace, int ssemodel, const cth_dpres& r, double* o) {
   __shared__ double G[kNSC * kScale], NG[kNSC * kScale], PJ[kNSC * kScale];
   __shared__ double pu[kNSC];
   __shared__ int left[kNSC];
   __shared__ double sh[8];  // umn, umx, E, H, lambda, K, total
-  const int k = blockIdx.x;
   const int lane = threadIdx.x;
-  const int p = plist[k];
-  const int* hc = hist + (int64_t)k * kNssc;
   if (lane < kNSC) {
     int l, r;
     grid_bounds(lane, &l, &r);
@@ -440,14 +444,45 @@ __global__ __launch_bounds__(64) void k_stats(const cth_dpset q, int qp, const c
     }
   }
   if (lane == 0) {
-    double* o = out + (int64_t)k * CTH_STATS_NOUT;
-    const cth_dpres r = dpres[prel[k]];
     double sig[4];
     significance(ssemodel, q.len[qp], db.len[p], (double)q.eno[qp], (double)db.eno[p], searchspace,
                  (double)r.score, (double)r.psts, lambda, K, sig);
     o[CTH_ST_LMBDEST] = sig[0]; o[CTH_ST_KEST] = sig[1]; o[CTH_ST_LOGE] = sig[2]; o[CTH_ST_LOGEPA] = sig[3];
     o[CTH_ST_MIN] = umn; o[CTH_ST_MAX] = umx; o[CTH_ST_E] = E; o[CTH_ST_H] = H;
     o[CTH_ST_LMBD] = lambda; o[CTH_ST_K] = K;
+  }
+}
+
+__global__ __launch_bounds__(64) void k_stats(const cth_dpset q, int qp, const cth_dpset db,
+                                              const int* __restrict__ plist, const int* __restrict__ prel,
+                                              const cth_dpres* __restrict__ dpres,
+                                              const int* __restrict__ hist, double searchspace, int ssemodel,
+                                              double* __restrict__ out) {
+  const int k = blockIdx.x;
+  stats_one(q, qp, db, plist[k], hist + (int64_t)k * kNssc, searchspace, ssemodel, dpres[prel[k]],
+            out + (int64_t)k * CTH_STATS_NOUT);
+}
+
+// Batched statistics over the *count passed pairs listed in list
+// (persistent, one wavefront per pair at a time): histogram in LDS, then the
+// parameters and significance against the pair's query's search space; out
+// and dpres are per pair.
+__global__ __launch_bounds__(64) void k_stats_batch(const cth_dpset q, const cth_dpset db,
+                                                    const cth_bquery* __restrict__ bq,
+                                                    const cth_bpair* __restrict__ pairs, const int* __restrict__ list,
+                                                    const int* __restrict__ count, int64_t ldsc,
+                                                    const float* __restrict__ scores,
+                                                    const cth_dpres* __restrict__ dpres, int ssemodel,
+                                                    double* __restrict__ out) {
+  __shared__ int h[kNssc];
+  const int nj = *count;
+  for (int k = blockIdx.x; k < nj; k += gridDim.x) {
+    const int i = list[k];
+    const cth_bpair pr = pairs[i];
+    const cth_bquery Q = bq[pr.b];
+    hist_fill(q, Q.qp, db, pr.p, db.dist[pr.p], ldsc, scores + Q.off, h);
+    stats_one(q, Q.qp, db, pr.p, h, Q.sspace, ssemodel, dpres[i], out + (int64_t)i * CTH_STATS_NOUT);
+    __syncthreads();
   }
 }
 
@@ -466,6 +501,19 @@ extern "C" int cth_launch_stats(const cth_dpset* q, int qp, const cth_dpset* db,
   CTH_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_stats, dim3(np2), dim3(64), 0, st, *q, qp, *db, plist_abs, plist_rel, dpres, hist,
                      searchspace, ssemodel, out);
+  CTH_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+// Batched statistics (k_stats_batch) on a persistent grid of `grid`
+// wavefronts over the *count pairs of list; out: CTH_STATS_NOUT doubles per pair.
+extern "C" int cth_launch_stats_batch(const cth_dpset* q, const cth_dpset* db, const cth_bquery* bq,
+                                      const cth_bpair* pairs, const int* list, const int* count, int64_t ldsc,
+                                      const float* scores, const cth_dpres* dpres, int ssemodel, double* out,
+                                      int grid, hipStream_t st) {
+  if (grid < 1) return 0;
+  hipLaunchKernelGGL(k_stats_batch, dim3(grid), dim3(64), 0, st, *q, *db, bq, pairs, list, count, ldsc, scores, dpres,
+                     ssemodel, out);
   CTH_HIP_CHECK(hipGetLastError());
   return 0;
 }

@@ -264,6 +264,26 @@ int cth_search(cth_ctx* c, int q0, int q1, const cth_search_opts* opt, cth_hit* 
 int cth_search_noalloc(cth_ctx* c, int q0, int q1, const cth_search_opts* opt, cth_hit* hits, int maxhits,
                        int* nhits, cth_search_stats* stats);
 
+/* The same search for many queries at once (no reference counterpart: the
+ * reference runs one query per ProcessScoreMatrix call, src/libmycu/cualn/
+ * DevCommThread.cpp:310-355, with the host choosing each stage's profiles).
+ * Consecutive queries whose buffers fit mem_budget together form a batch
+ * that runs each stage once over all its (query, profile) pairs in
+ * persistent launches, with the next stage's pairs selected on the device,
+ * and one synchronisation per batch; the launches are captured as a HIP
+ * graph and replayed while the batch shape stays the same, and the
+ * persistent grid sizes are autotuned per batch shape on its first run.
+ * Results (hits, their order, alignment rows) equal cth_search's.  Covers
+ * the FP32 contraction and databases whose profiles have at most 8192
+ * positions; other calls, and queries whose buffers alone exceed the budget,
+ * run as cth_search runs them.  With timing, the stages run without the
+ * graph and their times fill ms_k_score (score), ms_k_dp (both DP passes),
+ * ms_k_stats and ms_k_map; nchunks counts batches. */
+int cth_search_batch(cth_ctx* c, int q0, int q1, const cth_search_opts* opt, cth_hit* hits, int maxhits,
+                     int* nhits, cth_search_stats* stats);
+/* Graph captures and replays of cth_search_batch on a context. */
+int cth_batch_graph_counts(const cth_ctx* c, int64_t* captures, int64_t* replays);
+
 /* Length adjustment and effective search space of a query of qlen positions
  * against dbsize positions in ndb profiles with the gapped parameters
  * (AbstractScoreMatrix::ComputeLengthAdjustment, src/libpro/srcsco/

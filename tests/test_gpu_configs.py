@@ -99,14 +99,20 @@ def test_contraction_top100_on_batching_generators(ctx, big_sets, prec):
         assert len(a) >= 100 and len(b) >= 100, (qi, len(a), len(b))
         top_a, top_b = set(a["profile"][:100]), set(b["profile"][:100])
         shared = len(top_a & top_b)
-        common = np.intersect1d(a["profile"], b["profile"])
-        ea = np.array([a["logE"][a["profile"] == p][0] for p in common], np.float64)
-        eb = np.array([b["logE"][b["profile"] == p][0] for p in common], np.float64)
-        rel = np.abs(ea - eb) / np.maximum(1.0, np.abs(ea))
-        print(f"prec {prec} query {qi}: {len(a)} hits, top-100 shared {shared}, "
-              f"log E rel. difference max {rel.max():.2e} mean {rel.mean():.2e}", flush=True)
+        def logE_diff(profiles):
+            common = [p for p in profiles if p in set(b["profile"])]
+            ea = np.array([a["logE"][a["profile"] == p][0] for p in common], np.float64)
+            eb = np.array([b["logE"][b["profile"] == p][0] for p in common], np.float64)
+            return np.abs(ea - eb) / np.maximum(1.0, np.abs(ea))
+        top = logE_diff(a["profile"][:100])
+        every = logE_diff(a["profile"])
+        print(f"prec {prec} query {qi}: {len(a)} hits, top-100 shared {shared}, log E rel. difference "
+              f"top-100 max {top.max():.2e} mean {top.mean():.2e}; all hits max {every.max():.2e} "
+              f"mean {every.mean():.2e}", flush=True)
         assert shared >= 99, (qi, sorted(top_a ^ top_b))
-        assert rel.max() <= 0.05
+        # the top hits' E-values within 1e-2 relative (weak hits near E = 1
+        # move further: their log E is near 0 and shifts with the score)
+        assert top.max() <= 1e-2
     # the queries' copies are their first hits in both forms
     assert h32[0]["profile"] == 0 and hv[0]["profile"] == 0
 
