@@ -236,6 +236,13 @@ __global__ __launch_bounds__(64) void k_dp(cth_dpset q, int qp, cth_dpset db, in
 // scores (and writes its traceback bytes) at the query's offset; res is per
 // pair.  Pairs come in the order given (longest first), so the longest DPs
 // start first.
+//
+// No branch in this loop depends on the lane: every lane takes part in the
+// fetch (lane 0 adds 1, the others 0, so lane 0's old value is the pair) and
+// every lane stores the wave-uniform maximum.  A fetch under
+// `if (threadIdx.x == 0)` lets the compiler thread that loop-invariant branch
+// across the back edge, which leaves lanes 1..63 in a second copy of the loop
+// that never fetches again (they re-run the first pair forever).
 template <bool kTrace>
 __global__ __launch_bounds__(64) void k_dp_batch(cth_dpset q, cth_dpset db, const cth_bquery* __restrict__ bq,
                                                  const cth_bpair* __restrict__ pairs, const int* __restrict__ list,
@@ -245,9 +252,8 @@ __global__ __launch_bounds__(64) void k_dp_batch(cth_dpset q, cth_dpset db, cons
   extern __shared__ float lds_row[];
   const int nj = count ? *count : n;
   for (;;) {
-    int k = 0;
-    if (threadIdx.x == 0) k = atomicAdd(next, 1);
-    k = __builtin_amdgcn_readfirstlane(__shfl(k, 0));  // uniform: scalar loads of the pair and query
+    // uniform: scalar loads of the pair and query
+    const int k = __builtin_amdgcn_readfirstlane(atomicAdd(next, threadIdx.x == 0 ? 1 : 0));
     if (k >= nj) break;
     const int i = list ? list[k] : k;
     const cth_bpair pr = pairs[i];
@@ -256,10 +262,9 @@ __global__ __launch_bounds__(64) void k_dp_batch(cth_dpset q, cth_dpset db, cons
     uint32_t bestxy;
     dp_pair(q, Q.qp, db, pr.p, db.dist[pr.p], ldsc, scores + Q.off, kTrace ? btck + Q.off : nullptr, lds_row, best,
             bestxy);
-    if (threadIdx.x == 0) {
-      res[i].maxsc = best;
-      res[i].maxxy = bestxy;
-    }
+    // the butterfly in dp_pair leaves the same maximum in every lane
+    res[i].maxsc = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, best)));
+    res[i].maxxy = (uint32_t)__builtin_amdgcn_readfirstlane((int)bestxy);
   }
 }
 
