@@ -110,9 +110,12 @@ def test_contraction_top100_on_batching_generators(ctx, big_sets, prec):
               f"top-100 max {top.max():.2e} mean {top.mean():.2e}; all hits max {every.max():.2e} "
               f"mean {every.mean():.2e}", flush=True)
         assert shared >= 99, (qi, sorted(top_a ^ top_b))
-        # the top hits' E-values within 1e-2 relative (weak hits near E = 1
-        # move further: their log E is near 0 and shifts with the score)
-        assert top.max() <= 1e-2
+        # the top hits' log E-values within 8 unit roundoffs of the operand
+        # format relative (BF16 2^-8, FP16 2^-11; the per-cell error is about
+        # 2 of them), on average within one (weak hits near E = 1 move
+        # further: their log E is near 0 and shifts with the score)
+        u = 2.0 ** -8 if prec == 1 else 2.0 ** -11
+        assert top.max() <= 8 * u and top.mean() <= u
     # the queries' copies are their first hits in both forms
     assert h32[0]["profile"] == 0 and hv[0]["profile"] == 0
 
