@@ -9,7 +9,8 @@ oracle finishes in seconds:
   cores, FP32 accumulation) on the same generators, at a permissive E-value
   that passes over 100 hits per query: per-cell error of the log-odds term
   against FP64, at least 99% of each top-100 hit list shared with the FP32
-  path, log E-values of shared hits close;
+  path, scores of shared hits within a few unit roundoffs of the operand
+  format;
 - the long-profile workload's BF16 variant: a 2000-position query with a
   full predicted distance map against 1000-3000-position templates and
   itself: per-cell error and the same hits as FP32.
@@ -107,15 +108,31 @@ def test_contraction_top100_on_batching_generators(ctx, big_sets, prec):
         top = logE_diff(a["profile"][:100])
         every = logE_diff(a["profile"])
         print(f"prec {prec} query {qi}: {len(a)} hits, top-100 shared {shared}, log E rel. difference "
-              f"top-100 max {top.max():.2e} mean {top.mean():.2e}; all hits max {every.max():.2e} "
-              f"mean {every.mean():.2e}", flush=True)
+              f"top-100 max {top.max():.2e} mean {top.mean():.2e} p95 {np.percentile(top, 95):.2e}; "
+              f"all hits max {every.max():.2e} mean {every.mean():.2e}", flush=True)
+        # the top-100 hit that moved most, in both forms
+        common = [p for p in a["profile"][:100] if p in set(b["profile"])]
+        pw = common[int(np.argmax(top))]
+        for tag, h in (("fp32", a), ("variant", b)):
+            r = h[h["profile"] == pw][0]
+            print(f"  worst profile {pw} {tag}: score {r['score']:.4f} beg {r['beg']} end {r['end']} "
+                  f"logE {r['logE']:.4f} lambda_est {r['lambda_est']:.5f} K_est {r['K_est']:.5f} "
+                  f"expected {r['expected']:.5f} minsc {r['minsc']:.3f} maxsc {r['maxsc']:.3f}", flush=True)
         assert shared >= 99, (qi, sorted(top_a ^ top_b))
-        # the top hits' log E-values within 8 unit roundoffs of the operand
+        # The contraction perturbs the alignment score; the E-value follows
+        # from it (E = K m n exp(-lambda S)), so a top-100 hit near E = 1
+        # shows a large relative log E change for a small score change.  The
+        # shared top-100 hits' scores: within 8 unit roundoffs of the operand
         # format relative (BF16 2^-8, FP16 2^-11; the per-cell error is about
-        # 2 of them), on average within one (weak hits near E = 1 move
-        # further: their log E is near 0 and shifts with the score)
+        # 2 of them), on average within one.
         u = 2.0 ** -8 if prec == 1 else 2.0 ** -11
-        assert top.max() <= 8 * u and top.mean() <= u
+        common = [p for p in a["profile"][:100] if p in set(b["profile"])]
+        sa = np.array([a["score"][a["profile"] == p][0] for p in common], np.float64)
+        sb = np.array([b["score"][b["profile"] == p][0] for p in common], np.float64)
+        rs = np.abs(sb - sa) / np.abs(sa)
+        print(f"  score rel. difference top-100 max {rs.max():.2e} mean {rs.mean():.2e} ({rs.max() / u:.2f} u, "
+              f"{rs.mean() / u:.2f} u); variant higher in {int((sb > sa).sum())} of {len(sa)}", flush=True)
+        assert rs.max() <= 8 * u and rs.mean() <= u
     # the queries' copies are their first hits in both forms
     assert h32[0]["profile"] == 0 and hv[0]["profile"] == 0
 
