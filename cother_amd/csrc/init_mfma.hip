@@ -14,9 +14,9 @@
 // unchanged.  The FP32 path (the default) does not use this kernel.
 //
 // Tile: a workgroup of four wavefronts computes 64 query rows x 64 template
-// columns; each wavefront one 32 x 32 block with four
-// v_mfma_f32_32x32x8 (bf16 / f16) steps over K.  Operand layout of the 32x32x8
-// instruction (lane l, 64 lanes): A holds row l%32, k = 4*(l/32) + 0..3; B
+// columns; each wavefront one 32 x 32 block with two
+// v_mfma_f32_32x32x16 (bf16 / f16) steps over K.  Operand layout of the 32x32x16
+// instruction (lane l, 64 lanes): A holds row l%32, k = 8*(l/32) + 0..7; B
 // holds column l%32, the same k; the 16 accumulators of lane l hold column
 // l%32, rows 8*(r/4) + 4*(l/32) + r%4 for r = 0..15 (checked on the device by
 // cth_selftest_mfma).
@@ -27,8 +27,9 @@
 
 namespace {
 
-typedef short v4s __attribute__((ext_vector_type(4)));
-typedef _Float16 v4h __attribute__((ext_vector_type(4)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 
 constexpr int kTile = 64;
@@ -46,27 +47,25 @@ __device__ __forceinline__ uint16_t to_f16(float x) {
   return *reinterpret_cast<const uint16_t*>(&h);
 }
 
-template <int kPrec>  // 1: BF16, 2: FP16
+template <int kPrec>  // 1: BF16, 2: FP16; a, b: 8 consecutive k of the lane's row / column
 __device__ __forceinline__ v16f mfma(const uint16_t* a, const uint16_t* b, v16f acc) {
-  if constexpr (kPrec == 1) {
-    const v4s av = {(short)a[0], (short)a[1], (short)a[2], (short)a[3]};
-    const v4s bv = {(short)b[0], (short)b[1], (short)b[2], (short)b[3]};
-    return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, acc, 0, 0, 0);
-  } else {
-    v4h av, bv;
-    for (int k = 0; k < 4; k++) {
-      av[k] = *reinterpret_cast<const _Float16*>(&a[k]);
-      bv[k] = *reinterpret_cast<const _Float16*>(&b[k]);
-    }
-    return __builtin_amdgcn_mfma_f32_32x32x8f16(av, bv, acc, 0, 0, 0);
-  }
+  const v8s av = *reinterpret_cast<const v8s*>(a);
+  const v8s bv = *reinterpret_cast<const v8s*>(b);
+  if constexpr (kPrec == 1)
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(v8bf, av), __builtin_bit_cast(v8bf, bv), acc,
+                                                   0, 0, 0);
+  else
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(v8h, av), __builtin_bit_cast(v8h, bv), acc, 0,
+                                                  0, 0);
 }
 
 template <int kPrec>
 __global__ __launch_bounds__(256) void k_init_mfma(cth_dpset q, int qp, cth_dpset db, int64_t col0, int ncols,
                                                    int64_t ldsc, int lq, float* __restrict__ out) {
-  __shared__ uint16_t A[kTile][kK + 4];  // +4: spread the rows over LDS banks
-  __shared__ uint16_t B[kTile][kK + 4];
+  // +8: rows of 80 bytes, 16-byte aligned for the 8-element operand reads,
+  // spread over the LDS banks
+  __shared__ __align__(16) uint16_t A[kTile][kK + 8];
+  __shared__ __align__(16) uint16_t B[kTile][kK + 8];
   __shared__ float bq[CTH_NAA];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int row0 = blockIdx.y * kTile, c0 = blockIdx.x * kTile;
@@ -89,10 +88,12 @@ __global__ __launch_bounds__(256) void k_init_mfma(cth_dpset q, int qp, cth_dpse
   }
   __syncthreads();
   const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
-  const int l32 = lane & 31, kh = (lane >> 5) * 4;
+  // operands: row / column l32, k = kb + 8 (lane / 32) + 0..7; accumulators:
+  // rows 8 (r / 4) + 4 (lane / 32) + r % 4
+  const int l32 = lane & 31, kh = (lane >> 5) * 4, ko = (lane >> 5) * 8;
   v16f acc = {};
 #pragma unroll
-  for (int kb = 0; kb < kK; kb += 8) acc = mfma<kPrec>(&A[wr + l32][kb + kh], &B[wc + l32][kb + kh], acc);
+  for (int kb = 0; kb < kK; kb += 16) acc = mfma<kPrec>(&A[wr + l32][kb + ko], &B[wc + l32][kb + ko], acc);
   const int col = c0 + wc + l32;
   if (col >= ncols) return;
 #pragma unroll
@@ -102,17 +103,20 @@ __global__ __launch_bounds__(256) void k_init_mfma(cth_dpset q, int qp, cth_dpse
   }
 }
 
-// Lane layout check of the 32x32x8 instructions: A = I (32 x 8 block of the
-// identity), B[k][c] = 32 k + c (exact in BF16), so D[r][c] = B[r][c] for r < 8; out[lane*16 + r]
-// receives accumulator r of each lane.
+// Lane layout check of the 32x32x16 instructions: A = I (32 x 16 block of
+// the identity), B[k][c] = 32 (k - 8 h) + c for k in [8 h, 8 h + 8) and 0
+// elsewhere (h = 0, 1; every value below 256, exact in BF16), so
+// D[r][c] = B[r][c] for r < 16; out[lane*16 + r] receives accumulator r of
+// each lane.
 template <int kPrec>
-__global__ void k_selftest_mfma(float* out) {
+__global__ void k_selftest_mfma(float* out, int h) {
   const int lane = threadIdx.x;
-  const int l32 = lane & 31, kh = (lane >> 5) * 4;
-  uint16_t a[4], b[4];
-  for (int k = 0; k < 4; k++) {
-    const float av = (l32 == kh + k) ? 1.0f : 0.0f;
-    const float bv = (float)(32 * (kh + k) + l32);
+  const int l32 = lane & 31, ko = (lane >> 5) * 8;
+  __align__(16) uint16_t a[8], b[8];
+  for (int k = 0; k < 8; k++) {
+    const int kk = ko + k;
+    const float av = (l32 == kk) ? 1.0f : 0.0f;
+    const float bv = (kk >> 3) == h ? (float)(32 * (kk - 8 * h) + l32) : 0.0f;
     a[k] = kPrec == 1 ? to_bf16(av) : to_f16(av);
     b[k] = kPrec == 1 ? to_bf16(bv) : to_f16(bv);
   }
@@ -142,21 +146,22 @@ extern "C" int cth_selftest_mfma(int* bad) {
   CTH_HIP_CHECK(hipMalloc((void**)&d, 64 * 16 * sizeof(float)));
   float h[64 * 16];
   *bad = 0;
-  for (int prec = 1; prec <= 2; prec++) {
-    if (prec == 1) hipLaunchKernelGGL(k_selftest_mfma<1>, dim3(1), dim3(64), 0, 0, d);
-    else hipLaunchKernelGGL(k_selftest_mfma<2>, dim3(1), dim3(64), 0, 0, d);
-    if (hipMemcpy(h, d, sizeof h, hipMemcpyDeviceToHost) != hipSuccess) {
-      (void)hipFree(d);
-      cth_set_error("selftest_mfma: copy failed");
-      return -1;
-    }
-    for (int lane = 0; lane < 64; lane++)
-      for (int r = 0; r < 16; r++) {
-        const int row = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3), col = lane & 31;
-        const float want = row < 8 ? (float)(32 * row + col) : 0.0f;
-        if (h[lane * 16 + r] != want) (*bad)++;
+  for (int prec = 1; prec <= 2; prec++)
+    for (int half = 0; half < 2; half++) {
+      if (prec == 1) hipLaunchKernelGGL(k_selftest_mfma<1>, dim3(1), dim3(64), 0, 0, d, half);
+      else hipLaunchKernelGGL(k_selftest_mfma<2>, dim3(1), dim3(64), 0, 0, d, half);
+      if (hipMemcpy(h, d, sizeof h, hipMemcpyDeviceToHost) != hipSuccess) {
+        (void)hipFree(d);
+        cth_set_error("selftest_mfma: copy failed");
+        return -1;
       }
-  }
+      for (int lane = 0; lane < 64; lane++)
+        for (int r = 0; r < 16; r++) {
+          const int row = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3), col = lane & 31;
+          const float want = (row >> 3) == half ? (float)(32 * (row - 8 * half) + col) : 0.0f;
+          if (h[lane * 16 + r] != want) (*bad)++;
+        }
+    }
   (void)hipFree(d);
   return 0;
 }

@@ -46,20 +46,23 @@ __device__ __forceinline__ float ss_lookup(const float* ss, int row, int col) {
   return ss[col + 2];
 }
 
-// A segment's coordinates as two packed words: begins B = QB | PB << 16 and
-// ends E = (QE | PE << 16) + kEndBias.  Every index is below 256, so per half
-// E - B = 0x4000 + (end - begin) with no borrow between the halves, and bit
-// 14 of the half is set exactly when end >= begin.
+// A segment's coordinates as packed words: begin B = QB | PB << 16 and end
+// E = QE | PE << 16.  Every index is below 256, so with kEndBias = 0x4000 in
+// each half, (E + kEndBias) - B = 0x4000 + (end - begin) per half with no
+// borrow between the halves, and bit 14 of the half is set exactly when
+// end >= begin.  The register keeps each entry as nb = kEndBias - B and
+// ce = E + kEndBias, so that both differences of a test are one add or
+// subtract from a segment's B and E.
 constexpr uint32_t kEndBias = 0x40004000u;
 // entries that intersect nothing: begins 255, ends 0
-constexpr uint32_t kNoBegin = 0x00ff00ffu, kNoEnd = kEndBias;
+constexpr uint32_t kNoBeginN = kEndBias - 0x00ff00ffu, kNoEnd = kEndBias;
 
-// whether segment s (sb, se) intersects segment c (cb, ce): on the query or
-// the template axis the byte-wise max of the begins is <= the min of the
-// ends (CuBatchScoreMatrix_ddms2s.cuh:141-153), i.e. (begin <= end within
-// each segment) ce >= sb and se >= cb
-__device__ __forceinline__ bool coords_intersect(uint32_t sb, uint32_t se, uint32_t cb, uint32_t ce) {
-  return ((ce - sb) & (se - cb) & kEndBias) != 0u;
+// whether segment s (begin sb, end se) intersects register entry c (nb, ce):
+// on the query or the template axis the byte-wise max of the begins is <=
+// the min of the ends (CuBatchScoreMatrix_ddms2s.cuh:141-153), i.e. (begin
+// <= end within each segment) ce >= sb and se >= cb
+__device__ __forceinline__ bool meets(uint32_t sb, uint32_t se, uint32_t nb, uint32_t ce) {
+  return ((ce - sb) & (se + nb) & kEndBias) != 0u;
 }
 
 // RegisterMaxDDMScore (CuBatchScoreMatrix_ddms2s.cuh:168-261): keep the
@@ -74,33 +77,33 @@ __device__ __forceinline__ bool coords_intersect(uint32_t sb, uint32_t se, uint3
 // Only m0's and m1's coordinates are ever tested, so m2's are not kept.  An
 // entry whose value is 0 may keep stale coordinates: every intersection test
 // with it only decides between two zeros, and no segment intersects the
-// initial (kNoBegin, kNoEnd).  i0, i1: whether s intersects the current m0,
-// m1.  The register changes only if register_changes(); register_max is
-// called only then.
-__device__ __forceinline__ bool register_changes(float m0, float m1, float s, bool i0, bool i1) {
-  return (m0 <= s) | (!i0 & ((m1 <= s) | !i1));
+// initial entries (kNoBeginN, kNoEnd).  lt0: s < m0; i0, i1: whether s
+// intersects the current m0, m1.  The register changes only if
+// register_changes(); register_max is called only then.
+__device__ __forceinline__ bool register_changes(bool lt0, float m1, float s, bool i0, bool i1) {
+  return !lt0 | (!i0 & ((m1 <= s) | !i1));
 }
-__device__ __forceinline__ void register_max(float& m0, float& m1, float& m2, uint32_t& b0, uint32_t& e0,
-                                             uint32_t& b1, uint32_t& e1, float s, uint32_t sb, uint32_t se, bool i0,
-                                             bool i1) {
+__device__ __forceinline__ void register_max(float& m0, float& m1, float& m2, uint32_t& nb0, uint32_t& e0,
+                                             uint32_t& nb1, uint32_t& e1, float s, bool lt0, uint32_t sb,
+                                             uint32_t se, bool i0, bool i1) {
   // one block per case, so that a wavefront runs only the cases its lanes
   // have (most often only the first: the best segment growing)
-  if (m0 <= s) {
+  if (!lt0) {
     // s becomes m0
     const float ym1 = i1 ? 0.0f : m1;
     m2 = i0 ? 0.0f : ym1;
     m1 = i0 ? ym1 : m0;
-    b1 = i0 ? b1 : b0;
+    nb1 = i0 ? nb1 : nb0;
     e1 = i0 ? e1 : e0;
     m0 = s;
-    b0 = sb;
-    e0 = se;
+    nb0 = kEndBias - sb;
+    e0 = se + kEndBias;
   } else if (m1 <= s) {
     // s becomes m1 (s does not intersect m0 when the register changes)
     m2 = i1 ? 0.0f : m1;
     m1 = s;
-    b1 = sb;
-    e1 = se;
+    nb1 = kEndBias - sb;
+    e1 = se + kEndBias;
   } else {
     // s becomes m2 (s intersects neither m0 nor m1 when the register changes)
     m2 = s;
@@ -140,54 +143,67 @@ __device__ __forceinline__ void register_max(float& m0, float& m1, float& m2, ui
 // scom >= max(m2, FLT_MIN) (positive sums are far above FLT_MIN, so this is
 // scom > 0 && scom >= m2; m2 is finite, so the max is the median of m2,
 // FLT_MIN and FLT_MAX, one instruction).
-#define CTH_DDMS_CELL(X, DOUT, COUT)                                          \
+#define CTH_DDMS_CELL(X, COUT)                                                \
   {                                                                           \
     const float nz = __builtin_amdgcn_fmed3f(u * big, 0.0f, 1.0f);            \
     dd = fmaf(nz, dd, -nz);                                                   \
     const float t = fmaf(0.5f, u, X);                                         \
     u = t + fabsf(t);                                                         \
     COUT = u * __builtin_amdgcn_fmed3f(-dd - 4.0f, 0.0f, 1.0f);               \
-    DOUT = dd;                                                                \
   }
 // one 4-cell step, cells K0+1..K0+4 relative to the base ib, with pair
 // scores A2, A4 (cells K0+2 and K0+4) and the halves H0, H2, H4 of those of
 // cells K0 (the previous step's last cell), K0+2 and K0+4: the reference's
 // interpolated sum 0.5f * (a + b) is h(a) + h(b), halving being exact and
 // commuting with the rounding of the sum.  The begin and end of the step's
-// best are recovered from the cells' candidates only when it can be
-// registered.  Two levels: the step's best is a candidate when it reaches
-// max(m2, FLT_MIN); a candidate changes the register only when it passes the
-// intersection tests (most candidates intersect m0 or m1 while scoring below
-// them: segments of neighbouring diagonals through the same region), so the
+// best are recovered only when it can be registered.  Three levels: the
+// step's best is a candidate when it reaches max(m2, FLT_MIN) (m2 is zero
+// most of the time: a growing best segment drops it); a candidate whose
+// segment already meets m0 with its end at the step's first cell and that
+// scores below m0 is dropped after one intersection test, before its end cell
+// is selected (most candidates are of this kind: segments of neighbouring
+// diagonals through the same region); the rest take the full test, and the
 // update runs for a wavefront only when one of its lanes really changes its
 // register.  The empty asm statements keep the candidate's packed words in
 // registers: without them the compiler folds their sums into each test as
 // scalar-operand adds, which issue one per cycle.
 #define CTH_DDMS_4CELL(K0, H0, A2, H2, A4, H4, QOFF, POFF)                    \
   {                                                                           \
-    float cd1, cd2, cd3, cd4, dd1, dd2, dd3, dd4;                             \
-    CTH_DDMS_CELL(H0 + H2, dd1, cd1)                                          \
-    CTH_DDMS_CELL(A2, dd2, cd2)                                               \
-    CTH_DDMS_CELL(H2 + H4, dd3, cd3)                                          \
-    CTH_DDMS_CELL(A4, dd4, cd4)                                               \
+    float cd1, cd2, cd3, cd4;                                                 \
+    CTH_DDMS_CELL(H0 + H2, cd1)                                               \
+    const float dd1 = dd;                                                     \
+    CTH_DDMS_CELL(A2, cd2)                                                    \
+    CTH_DDMS_CELL(H2 + H4, cd3)                                               \
+    CTH_DDMS_CELL(A4, cd4)                                                    \
     const float scom = fmaxf(fmaxf(cd1, cd2), fmaxf(cd3, cd4));               \
     if (scom >= m2f) {                                                        \
-      uint32_t kk = 4u * 0x10001u;                                            \
-      float dk = dd4;                                                         \
-      kk = cd3 == scom ? 3u * 0x10001u : kk; dk = cd3 == scom ? dd3 : dk;     \
-      kk = cd2 == scom ? 2u * 0x10001u : kk; dk = cd2 == scom ? dd2 : dk;     \
-      kk = cd1 == scom ? 0x10001u : kk; dk = cd1 == scom ? dd1 : dk;          \
-      /* end (ib + K0 + k) and begin (end + dk) on both axes, packed */       \
-      uint32_t se = kk + ((uint32_t)(ib + (K0)) * 0x10001u + (uint32_t)(QOFF) + \
-                          ((uint32_t)(POFF) << 16) + kEndBias);               \
-      asm("" : "+v"(se));                                                     \
-      uint32_t sb = (se - kEndBias) + (uint32_t)(int)(dk * 65537.0f);         \
+      /* the step's first cell on both axes, packed */                        \
+      const uint32_t base = (uint32_t)(ib + (K0) + 1) * 0x10001u +            \
+                            (uint32_t)(QOFF) + ((uint32_t)(POFF) << 16);      \
+      /* begin: the candidate's segment has DSTSEGM or more cells at a cell   \
+         of this step, so it runs at the step's first cell, whose dd gives    \
+         its begin */                                                         \
+      uint32_t sb = base + (uint32_t)__mul24((int)dd1, 0x10001);              \
       asm("" : "+v"(sb));                                                     \
-      const bool i0 = coords_intersect(sb, se, cb0, ce0);                     \
-      const bool i1 = coords_intersect(sb, se, cb1, ce1);                     \
-      if (register_changes(m0, m1, scom, i0, i1)) {                           \
-        register_max(m0, m1, m2, cb0, ce0, cb1, ce1, scom, sb, se, i0, i1);   \
-        m2f = __builtin_amdgcn_fmed3f(m2, FLT_MIN, FLT_MAX);                  \
+      /* the end lies in cells 1..4 and an intersection, once there, stays as \
+         the end grows: a segment that meets m0 ending at cell 1 meets it at  \
+         its end, and below m0 it cannot change the register */               \
+      const bool lt0 = scom < m0;                                             \
+      if (!(meets(sb, base, nb0, ce0) && lt0)) {                              \
+        uint32_t kk = 3u * 0x10001u;                                          \
+        kk = cd3 == scom ? 2u * 0x10001u : kk;                                \
+        kk = cd2 == scom ? 0x10001u : kk;                                     \
+        kk = cd1 == scom ? 0u : kk;                                           \
+        /* end (ib + K0 + k) on both axes, packed */                          \
+        uint32_t se = kk + base;                                              \
+        asm("" : "+v"(se));                                                   \
+        const bool i0 = meets(sb, se, nb0, ce0);                              \
+        const bool i1 = meets(sb, se, nb1, ce1);                              \
+        if (register_changes(lt0, m1, scom, i0, i1)) {                        \
+          register_max(m0, m1, m2, nb0, ce0, nb1, ce1, scom, lt0, sb, se, i0, \
+                       i1);                                                   \
+          m2f = __builtin_amdgcn_fmed3f(m2, FLT_MIN, FLT_MAX);                \
+        }                                                                     \
       }                                                                       \
     }                                                                         \
   }
@@ -210,7 +226,7 @@ __device__ __forceinline__ float ddms_cell(const float* __restrict__ dmsT, const
   const int len1 = q[0], len2 = t[0];
   if (len1 < CTH_DSTSEGM || len2 < CTH_DSTSEGM) return 0.0f;
   float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f, m2f = FLT_MIN;
-  uint32_t cb0 = kNoBegin, ce0 = kNoEnd, cb1 = kNoBegin, ce1 = kNoEnd;
+  uint32_t nb0 = kNoBeginN, ce0 = kNoEnd, nb1 = kNoBeginN, ce1 = kNoEnd;
   // 2^126 in a vector register: as a scalar operand it would keep the
   // multiply off the second issue slot
   float big;
