@@ -33,7 +33,7 @@
 // fastest candidate is kept for that shape (database size and pair count).
 //
 // Covered: the FP32 contraction and databases whose profiles fit the LDS
-// boundary rows (CTH_LDS_ROW_MAX positions); other searches, and queries
+// boundary rows (CTH_DP_LDS_ROW_MAX positions); other searches, and queries
 // whose buffers alone exceed the budget, run through cth_search.
 #include <hip/hip_runtime.h>
 
@@ -275,7 +275,7 @@ extern "C" int cth_search_batch(cth_ctx* c, int q0, int q1, const cth_search_opt
   cth_search_stats st{};
   // the batched form covers the FP32 contraction and profiles within the LDS
   // row limit; otherwise the queries run through cth_search
-  if (opt->contraction != 0 || ltmax > CTH_LDS_ROW_MAX || nprof == 0) {
+  if (opt->contraction != 0 || ltmax > CTH_DP_LDS_ROW_MAX || nprof == 0) {
     if (cth_search(c, q0, q1, opt, hits, maxhits, nhits, stats)) return -1;
     return 0;
   }

@@ -18,6 +18,9 @@
 // longest profile whose DP/MAP stripe boundary rows (5 floats per column)
 // fit one workgroup's 160 KiB of LDS; longer ones keep them in global memory
 #define CTH_LDS_ROW_MAX 8192
+// the phase-1 DP keeps the reference's buffer of maxima (value and cell per
+// column) next to its five boundary rows: 28 bytes per column in LDS
+#define CTH_DP_LDS_ROW_MAX 5824
 
 // transition order (cudp/CuBatchDP_com.h:87-96)
 enum { CTH_MM = 0, CTH_MI, CTH_MD, CTH_IM, CTH_II, CTH_DM, CTH_DD };

@@ -9,16 +9,19 @@
 // DPP wave_shr:1 move, so the wavefront advances one anti-diagonal per step
 // with no LDS round trip.  The bottom row of a stripe is kept in LDS for the
 // next stripe (in a global-memory row of the wavefront's own for profiles
-// longer than CTH_LDS_ROW_MAX positions, whose five rows exceed the LDS; the
-// row of column x is read at step x + 31 and overwritten at step x + 95 of the
-// same wavefront, so one row serves both stripes as in LDS).  Cell values
+// longer than CTH_DP_LDS_ROW_MAX positions, whose five rows and buffer of
+// maxima exceed the LDS; the row of column x is read at step x + 31 and
+// overwritten at step x + 95 of the same wavefront, so one row serves both
+// stripes as in LDS).  Cell values
 // equal the reference's: rows start at column -1-(q mod 32) with zero
 // boundaries, rows q = 0 mod 32 see zeros above at
 // negative columns, rows q = 31 mod 32 see zeros left and upper-left at
 // column 0, out-of-range transitions are log(0) = -32768 and cells
 // outside the valid area score -9 -- the boundaries the reference's 32x32
-// oblique blocks produce.  The maximum is the first maximum in row-major
-// order (per-lane strict max, then a (value, row, column) reduction).
+// oblique blocks produce.  The maximum and its cell are the ones the
+// reference's per-column buffer of maxima yields (dp_pair below), so equal
+// maxima resolve as there, and for templates under 32 positions the rows
+// whose block-end column lies before the matrix drop out as there.
 //
 // k_dpfin restates FinalizeDPPro (cudp/CuBatchDP_final.cu:61-292): pass
 // test against the score threshold, traceback from the maximum, correlation
@@ -52,10 +55,41 @@ __device__ __forceinline__ bool in_valid_area(int lq, int lt, int edge, int y, i
   return !corners && 0 <= y && y < lq && 0 <= x && x < lt;
 }
 
+// MaxIterationsForBlock: the columns the reference's block starting at x0
+// in the block row ending at query row y iterates over
+__device__ __forceinline__ int block_iters(int x0, int y, int lq, int lt) {
+  int x = lt - x0;
+  if (x <= 0) return 0;
+  const int yy = y - (lq - 1);
+  if (0 < yy) x -= yy;
+  if (x <= 0) return 0;
+  return x < 32 ? x : 32;
+}
+
 // The DP of query qp against template profile p, whose scores begin at
 // scores[colp] (row stride ldsc); traceback bytes to btck at the same offsets
-// (NULL: none); bot: the stripe boundary rows [5][lt] (LDS or global).  The
-// first maximum in row-major order and its coordinates reach every lane.
+// (NULL: none); bot: [7][lt] (LDS or global): the stripe boundary rows [5]
+// and the reference's buffer of maxima, values and cells.  The maximum and
+// its cell follow that buffer (see below) and reach every lane.
+//
+// The reference (ExecDP_Btck_Unroll32x, cudp/CuBatchDP_init_btck.cu:246-263,
+// 328, 436-464) computes 32x32 oblique blocks; thread tx of a block starting
+// at column x0 in the block row of query rows 32b..32b+31 takes row
+// 32b+31-tx, columns x0+tx .. x0+tx+31, keeps the first cell of a larger MM
+// value (strict <), seeded from the buffer at column x0+tx-1, and at the
+// block's end writes the buffer at column x0+tx+ndx (ndx = 31, but 0 for a
+// block whose column x0+31 is past the end and lt-x0-32 for the block that
+// covers the end; block row 0 overwrites, later ones keep the buffer's
+// value unless it is smaller).  FinalizeDPPro (cudp/CuBatchDP_final.cu:
+// 100-142) reduces the last 64 columns.  Here lane l of stripe s is row
+// 64s+l of block row 2s + l/32 with tx = 31 - l%32, and it reaches column c at
+// step c + l + 32: every lane's blocks start at steps = 31 and end at steps
+// = 30 (mod 32).  At a block end the lower half-wavefront writes before the
+// upper one (its block row's blocks run two launches ahead, so where the
+// right-edge writes of both meet a column, the lower one's comes first);
+// every buffer column then sees its block rows' writes and reads in the
+// reference's order.  Lanes past the query's end take part as the
+// reference's threads of those rows do.
 __device__ __forceinline__ void dp_pair(const cth_dpset& q, int qp, const cth_dpset& db, int p, int64_t colp,
                                         int64_t ldsc, const float* __restrict__ scores, uint8_t* __restrict__ btck,
                                         float* bot, float& best_out, uint32_t& bestxy_out) {
@@ -64,14 +98,21 @@ __device__ __forceinline__ void dp_pair(const cth_dpset& q, int qp, const cth_dp
   const int64_t qb = q.dist[qp] + qp;      // query transition vector 0
   const int64_t tb = db.dist[p] + p;       // template transition vector 0
   const int edge = corner_maxlen(lq, lt);
-  const int ltmax = lt;  // row stride of bot (the launch reserves 5 x its longest profile per wavefront)
-  float best = 0.0f;
-  uint32_t bestxy = 0;
+  const int ltmax = lt;  // row stride of bot (the launch reserves 7 x its longest profile per wavefront)
+  float* const bmv = bot + 5 * ltmax;
+  uint32_t* const bmc = reinterpret_cast<uint32_t*>(bot + 6 * ltmax);
   const int nstripes = (lq + 63) >> 6;
   const int c0 = -1 - (lane & 31);
+  const int hi = lane & 32;           // 32 for the upper half-wavefront
+  const int tx = 31 - (lane & 31);    // the reference's thread in its block
   for (int s = 0; s < nstripes; s++) {
     const int y = (s << 6) + lane;
     const bool rowok = y < lq;
+    const int band = 2 * s + (hi >> 5);     // block row of this lane
+    const int yb = 32 * band + 31;          // its last query row
+    const bool bandok = 32 * band < lq;     // the block row takes part
+    float mv = 0.0f;                        // the lane's running maximum
+    uint32_t mc = 0;                        // and its cell
     float qt[CTH_NTRN];
     if (rowok) {
       qt[CTH_MM] = q.trn[CTH_MM * q.ldtr + qb + y];
@@ -126,10 +167,25 @@ __device__ __forceinline__ void dp_pair(const cth_dpset& q, int qp, const cth_dp
     auto load_sc = [&](int c) { return (rowok && 0 <= c && c < lt) ? srow[c] : 0.0f; };
     load_col(-32, ntn, bn);
     float sc1 = load_sc(-lane - 32), sc2 = load_sc(-lane - 31);
-    const int nsteps = lt + 95;
+    // the last block of the upper half ends at step 32 ceil(lt/32) + 94
+    const int nsteps = 32 * ((lt + 31) >> 5) + 95;
     for (int t = 0; t < nsteps; t++) {
       const int c = t - lane - 32;
       const int cu = t - 32;
+      if ((t & 31) == 31) {
+        // a block starts: seed from the buffer at the column before
+        const int x0 = t - 63 - hi;
+        if (bandok && -32 <= x0 && x0 < lt) {
+          const int xs = x0 + tx - 1;
+          if (0 <= xs && xs < lt) {
+            mv = bmv[xs];
+            mc = bmc[xs];
+          } else {
+            mv = 0.0f;
+            mc = 0u;
+          }
+        }
+      }
       // this step's column data, then the loads for the steps after
       float nt[CTH_NTRN], bcu[5];
 #pragma unroll
@@ -186,7 +242,7 @@ __device__ __forceinline__ void dp_pair(const cth_dpset& q, int qp, const cth_dp
         if (m < nND) { m = nND; bk = CTH_BT_LEFT; }
         cMM = nMM; cMI = nMI; cIM = nIM; cDN = nDN; cND = nND;
         if (0 <= c) {
-          if (best < nMM) { best = nMM; bestxy = ((uint32_t)c << 16) | (uint32_t)y; }
+          if (mv < nMM) { mv = nMM; mc = ((uint32_t)c << 16) | (uint32_t)y; }
           if (btck) btck[(int64_t)y * ldsc + colp + c] = (uint8_t)bk;
         }
       }
@@ -195,19 +251,46 @@ __device__ __forceinline__ void dp_pair(const cth_dpset& q, int qp, const cth_dp
         bot[0 * ltmax + c] = cMM; bot[1 * ltmax + c] = cMI; bot[2 * ltmax + c] = cIM;
         bot[3 * ltmax + c] = cDN; bot[4 * ltmax + c] = cND;
       }
+      if ((t & 31) == 30) {
+        // a block ends: write the buffer, the lower half-wavefront first
+        const int x0 = t - 94 - hi;
+        int ndx = 31;
+        const int xtl = x0 + 31;
+        if (lt <= xtl) ndx = 0;
+        else if (lt <= xtl + ndx) ndx = lt - xtl - 1;
+        const int w = x0 + tx + ndx;
+        const bool wr = bandok && -32 <= x0 && x0 < lt && block_iters(x0, yb, lq, lt) >= 1 && 0 <= w && w < lt;
+        if (wr && hi == 0 && (band == 0 || bmv[w] < mv)) {
+          bmv[w] = mv;
+          bmc[w] = mc;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        if (wr && hi != 0 && bmv[w] < mv) {
+          bmv[w] = mv;
+          bmc[w] = mc;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+      }
     }
     __syncthreads();
   }
-  // first maximum in row-major order across the wavefront
-  for (int off = 32; off >= 1; off >>= 1) {
-    const float ob = __shfl_xor(best, off);
-    const uint32_t oxy = __shfl_xor(bestxy, off);
-    const uint32_t my = bestxy & 0xffff, oy = oxy & 0xffff;
-    const bool take = ob > best || (ob == best && (oy < my || (oy == my && (oxy >> 16) < (bestxy >> 16))));
-    if (take) { best = ob; bestxy = oxy; }
+  // FinalizeDPPro: column lt-1-l against lt-33-l, then halves of 32 .. 1, a
+  // tie keeping the lower index (the later column)
+  float v = lane < lt ? bmv[lt - 1 - lane] : 0.0f;
+  int ix = lane;
+  {
+    const float o = __shfl_down(v, 32);
+    if (lane < 32 && v < o) { v = o; ix = lane + 32; }
   }
-  best_out = best;
-  bestxy_out = bestxy;
+  for (int h = 16; h >= 1; h >>= 1) {
+    const float o = __shfl_down(v, h);
+    const int oi = __shfl_down(ix, h);
+    if (lane < 32 && v < o) { v = o; ix = oi; }
+  }
+  const float vb = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+  const int ib = __builtin_amdgcn_readfirstlane(ix);
+  best_out = vb;
+  bestxy_out = bmc[lt - 1 - (ib < lt ? ib : lt - 1)];
 }
 
 template <bool kGlobalRow>
@@ -216,8 +299,8 @@ __global__ __launch_bounds__(64) void k_dp(cth_dpset q, int qp, cth_dpset db, in
                                            const float* __restrict__ scores, uint8_t* __restrict__ btck,
                                            cth_dpres* __restrict__ res, float* __restrict__ grow,
                                            int64_t growstride) {
-  // [5][lt]: bottom row of the previous stripe, in LDS or in this
-  // wavefront's growstride floats of grow
+  // [7][lt]: bottom row of the previous stripe and the buffer of maxima, in
+  // LDS or in this wavefront's growstride floats of grow
   extern __shared__ float lds_row[];
   float* const bot = kGlobalRow ? grow + (int64_t)blockIdx.x * growstride : lds_row;
   const int p = plist ? plist[blockIdx.x] : pbase + blockIdx.x;
@@ -374,14 +457,14 @@ extern "C" int cth_selftest_dpp(int* host_out64) {
 // Bytes of global boundary rows a launch over np profiles of at most ltmax
 // positions needs (0 when the rows fit the LDS).
 extern "C" size_t cth_dp_row_bytes(int np, int ltmax) {
-  return ltmax > CTH_LDS_ROW_MAX ? sizeof(float) * 5 * (size_t)ltmax * (size_t)(np > 0 ? np : 0) : 0;
+  return ltmax > CTH_DP_LDS_ROW_MAX ? sizeof(float) * 7 * (size_t)ltmax * (size_t)(np > 0 ? np : 0) : 0;
 }
 
 // Phase 1 for np profiles: p0 + k, or plist[k] (device array of absolute
 // indices) when plist is not NULL; res[k] receives profile k's record.
 // btck NULL: no traceback bytes (maximum and its coordinates only);
 // finalize: run the finalisation (needs btck).  grow: cth_dp_row_bytes(np,
-// ltmax) bytes of device memory when ltmax exceeds CTH_LDS_ROW_MAX (else
+// ltmax) bytes of device memory when ltmax exceeds CTH_DP_LDS_ROW_MAX (else
 // unused).
 extern "C" int cth_launch_dp_list(const cth_dpset* q, int qp, const cth_dpset* db, int p0, const int* plist, int np,
                                   int ltmax, int64_t col0, int64_t ldsc, const float* scores, uint8_t* btck,
@@ -389,12 +472,12 @@ extern "C" int cth_launch_dp_list(const cth_dpset* q, int qp, const cth_dpset* d
   if (np <= 0) return 0;
   if (finalize && !btck) { cth_set_error("dp: finalisation needs the traceback"); return -1; }
   if (ltmax > CTH_MAXCOLUMNS) { cth_set_error("dp: profile length %d over %d", ltmax, CTH_MAXCOLUMNS); return -1; }
-  if (ltmax > CTH_LDS_ROW_MAX) {
+  if (ltmax > CTH_DP_LDS_ROW_MAX) {
     if (!grow) { cth_set_error("dp: profile length %d needs global boundary rows", ltmax); return -1; }
     hipLaunchKernelGGL(k_dp<true>, dim3(np), dim3(64), 0, st, *q, qp, *db, p0, plist, col0, ldsc, scores, btck, res,
-                       grow, (int64_t)5 * ltmax);
+                       grow, (int64_t)7 * ltmax);
   } else {
-    const size_t lds = sizeof(float) * 5 * (size_t)(ltmax > 0 ? ltmax : 1);
+    const size_t lds = sizeof(float) * 7 * (size_t)(ltmax > 0 ? ltmax : 1);
     hipLaunchKernelGGL(k_dp<false>, dim3(np), dim3(64), lds, st, *q, qp, *db, p0, plist, col0, ldsc, scores, btck,
                        res, nullptr, (int64_t)0);
   }
@@ -422,9 +505,9 @@ extern "C" int cth_launch_dp_batch(const cth_dpset* q, const cth_dpset* db, cons
                                    const cth_bpair* pairs, const int* list, const int* count, int n, int* next,
                                    int ltmax, int64_t ldsc, const float* scores, uint8_t* btck, cth_dpres* res,
                                    int grid, hipStream_t st) {
-  if (ltmax > CTH_LDS_ROW_MAX) { cth_set_error("dp batch: profile length %d over the LDS row limit", ltmax); return -1; }
+  if (ltmax > CTH_DP_LDS_ROW_MAX) { cth_set_error("dp batch: profile length %d over the LDS row limit", ltmax); return -1; }
   if (grid < 1) return 0;
-  const size_t lds = sizeof(float) * 5 * (size_t)(ltmax > 0 ? ltmax : 1);
+  const size_t lds = sizeof(float) * 7 * (size_t)(ltmax > 0 ? ltmax : 1);
   if (btck)
     hipLaunchKernelGGL(k_dp_batch<true>, dim3(grid), dim3(64), lds, st, *q, *db, bq, pairs, list, count, n, next, ldsc,
                        scores, btck, res);

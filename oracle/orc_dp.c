@@ -23,12 +23,12 @@
  *     cell outside the valid area (outside the matrix or in the two ignored
  *     corner triangles, CellYXinValidArea, cudp/CuBatchDP_init_btck.cuh:118-170)
  *     is -9 (:241-251).
- * The maximum MM value is tracked over the cells of the matrix; the
- * reference's max/argmax flows through a per-column buffer and is reduced by
- * FinalizeDPPro (cudp/CuBatchDP_final.cu:122-142) -- for equal maxima its
- * winner depends on that block schedule.  This restatement (and the HIP
- * path) takes the first maximum in row-major order; only exact float ties
- * between distinct cells can differ.
+ * The maximum MM value and its cell follow the reference's flow through its
+ * per-column buffer, so that equal maxima resolve as there (ref_max below):
+ * each thread of a 32x32 block keeps the running maximum of the cells it
+ * computes, seeded from the buffer at the column before its first cell and
+ * written back at the block's end (:452-480), and FinalizeDPPro
+ * (cudp/CuBatchDP_final.cu:100-142) reduces the buffer's last 64 columns.
  *
  * FinalizeDPPro (cudp/CuBatchDP_final.cu:61-292) is restated for the
  * pass decision, the traceback, the correlation bonus (32-score chunks
@@ -76,6 +76,94 @@ static void trn_at(const orc_pset* s, int prof, int pos, int len, float out[7]) 
 
 static inline float mx(float a, float b) { return a < b ? b : a; }
 
+/* MaxIterationsForBlock (cudp/CuBatchDP_init_btck.cuh): the columns a block
+ * starting at x0 in the block row ending at query row y iterates over */
+static int block_iters(int x0, int y, int lq, int lt) {
+  int x = lt - x0;
+  if (x <= 0) return 0;
+  const int yy = y - (lq - 1);
+  if (0 < yy) x -= yy;
+  if (x <= 0) return 0;
+  return x < 32 ? x : 32;
+}
+
+/* The reference's maximum of the MM values mm [lq][lt] and its cell
+ * (x << 16 | y).  A block row (query rows 32b .. 32b+31) runs its blocks
+ * x0 = -32, 0, 32, ... left to right; thread tx of a block takes query row
+ * 32b+31-tx, columns x0+tx .. x0+tx+iters-1.  At the block's start each
+ * thread reads the buffer at column x0+tx-1 (zero when outside [0, lt)), it
+ * keeps the first cell of a larger value (strict <), and at the block's end
+ * it writes the buffer at column x0+tx+ndx, ndx = 31 but 0 for a block
+ * whose last column x0+31 is past the end and lt-x0-32 for the block that
+ * covers the end (:452-480); the first block row overwrites, later ones keep
+ * the buffer's value unless it is smaller.  For every buffer column the
+ * block rows' writes and reads alternate top to bottom, so block rows are
+ * replayed one after another.  FinalizeDPPro then reduces the last 64
+ * columns pairwise (column lt-1-t against lt-33-t), then over halves of
+ * 32 .. 1, a tie keeping the lower index, i.e. the later column. */
+static void ref_max(const float* mm, int lq, int lt, float* maxsc, uint32_t* maxxy) {
+  float* buf = (float*)calloc((size_t)lt, sizeof(float));
+  uint32_t* bco = (uint32_t*)calloc((size_t)lt, sizeof(uint32_t));
+  for (int b = 0; 32 * b < lq; b++) {
+    const int y = 32 * b + 31;
+    for (int x0 = -32; x0 < lt; x0 += 32) {
+      const int iters = block_iters(x0, y, lq, lt);
+      if (iters < 1) continue;
+      float ms[32];
+      uint32_t mc[32];
+      for (int tx = 0; tx < 32; tx++) {
+        const int xp = x0 + tx - 1;
+        ms[tx] = (0 <= xp && xp < lt) ? buf[xp] : 0.0f;
+        mc[tx] = (0 <= xp && xp < lt) ? bco[xp] : 0u;
+      }
+      for (int tx = 0; tx < 32; tx++) {
+        const int r = y - tx;
+        if (r < 0 || lq <= r) continue;
+        for (int i = 0; i < iters; i++) {
+          const int c = x0 + tx + i;
+          if (c < 0 || lt <= c) continue;
+          const float v = mm[(size_t)r * lt + c];
+          if (ms[tx] < v) { ms[tx] = v; mc[tx] = ((uint32_t)c << 16) | (uint32_t)r; }
+        }
+      }
+      int ndx = 31;
+      const int xtl = x0 + 31;
+      if (lt <= xtl) ndx = 0;
+      else if (lt <= xtl + ndx) ndx = lt - xtl - 1;
+      for (int tx = 0; tx < 32; tx++) {
+        const int w = x0 + tx + ndx;
+        if (w < 0 || lt <= w) continue;
+        if (b == 0 || buf[w] < ms[tx]) { buf[w] = ms[tx]; bco[w] = mc[tx]; }
+      }
+    }
+  }
+  float v[64];
+  int ix[64];
+  for (int t = 0; t < 32; t++) {
+    v[t] = t < lt ? buf[lt - 1 - t] : 0.0f;
+    v[t + 32] = t + 32 < lt ? buf[lt - 33 - t] : 0.0f;
+  }
+  for (int t = 0; t < 32; t++) {
+    ix[t] = t;
+    if (v[t] < v[t + 32]) { v[t] = v[t + 32]; ix[t] = t + 32; }
+  }
+  for (int h = 16; h >= 1; h >>= 1) {
+    float nv[32];
+    int ni[32];
+    for (int t = 0; t < 32; t++) {
+      nv[t] = v[t];
+      ni[t] = ix[t];
+      if (t + h < 32 && v[t] < v[t + h]) { nv[t] = v[t + h]; ni[t] = ix[t + h]; }
+    }
+    memcpy(v, nv, sizeof nv);
+    memcpy(ix, ni, sizeof ni);
+  }
+  *maxsc = v[0];
+  *maxxy = bco[lt - 1 - ix[0]];
+  free(buf);
+  free(bco);
+}
+
 /* one template profile: DP with traceback bytes and the max */
 static void dp_profile(const orc_pset* q, int qp, const orc_pset* db, int tp, const float* sc, int64_t ldsc,
                        int64_t col, uint8_t* bt, float* maxsc, uint32_t* maxxy) {
@@ -86,8 +174,7 @@ static void dp_profile(const orc_pset* q, int qp, const orc_pset* db, int tp, co
   float* cur = (float*)calloc((size_t)W * NST, sizeof(float));
   float* dtr = (float*)malloc(sizeof(float) * 7 * (size_t)W);
   for (int c = -33; c < lt; c++) trn_at(db, tp, c, lt, dtr + 7 * (c + 33));
-  float best = 0.0f;
-  uint32_t bestxy = 0;
+  float* mmv = (float*)calloc((size_t)lq * (size_t)lt, sizeof(float));
   for (int y = 0; y < lq; y++) {
     float qt[7];
     trn_at(q, qp, y, lq, qt);
@@ -119,7 +206,7 @@ static void dp_profile(const orc_pset* q, int qp, const orc_pset* db, int tp, co
       O[MM] = v1;
       m = v1;
       bk = v1 ? BT_DIAG : BT_STOP;
-      if (0 <= c && best < v1) { best = v1; bestxy = ((uint32_t)c << 16) | (uint32_t)y; }
+      if (0 <= c) mmv[(size_t)y * lt + c] = v1;
       v1 = U[MM] + qt[tMM] + dt[tMI];
       v2 = U[MI] + qt[tMM] + dt[tII];
       O[MI] = mx(v1, v2);
@@ -140,9 +227,8 @@ static void dp_profile(const orc_pset* q, int qp, const orc_pset* db, int tp, co
     }
     float* t = prev; prev = cur; cur = t;
   }
-  *maxsc = best;
-  *maxxy = bestxy;
-  free(prev); free(cur); free(dtr);
+  ref_max(mmv, lq, lt, maxsc, maxxy);
+  free(mmv); free(prev); free(cur); free(dtr);
 }
 
 /* butterfly sum of 32 lane values with xor offsets 16,8,4,2,1; lane 0's result */

@@ -179,3 +179,81 @@ def simulate(S, qtrn, qbase, qld, lq, dtrn, dbase, dld, lt):
                 ms[tx] = old_ms[tx + i]; nd_[tx] = old_nd[tx + i]
         i >>= 1
     return btck, ms[0], int(mco[lt - 1 - nd_[0]])
+
+
+def flow_max(mm, lq, lt):
+    """The reference's maximum and its cell (x << 16 | y) for given MM values
+    mm [lq][lt]: the per-thread running maximum and the per-column buffer of
+    ExecDP_Btck_Unroll32x (cudp/CuBatchDP_init_btck.cu:246-263, 328, 436-464)
+    in the launch order of the block diagonals (as simulate() replays them),
+    then FinalizeDPPro's reduction of the last 64 columns
+    (cudp/CuBatchDP_final.cu:100-142)."""
+    diagm = np.zeros(lt, f32)
+    mco = np.zeros(lt, np.uint32)
+    nblk = ((lt + lq) + 31) // 32 + (lq - 1) // 32
+    last = 2 * ((lq - 1) // 32)
+    for d in range(nblk):
+        reads, writes = [], []
+        for bx in range(nblk + 2):
+            if d <= last:
+                x0 = (2 * bx - (0 if d & 1 else 1)) * D
+                y = ((d >> 1) + 1 - bx) * D - 1
+            else:
+                x0 = (2 * bx + (d - last - 1)) * D
+                y = ((last >> 1) + 1 - bx) * D - 1
+            ilim = _maxiters(x0, y, lq, lt)
+            if y < 0 or lq <= y + 1 - D or lt <= x0 or ilim < 1:
+                continue
+            mxs = [f32(0)] * D
+            mxc = [0] * D
+            for tx in range(D):
+                x = x0 + tx
+                if 0 <= x - 1 < lt:
+                    mxs[tx] = diagm[x - 1]
+                    mxc[tx] = int(mco[x - 1])
+            for tx in range(D):
+                r = y - tx
+                for i in range(ilim):
+                    c = x0 + tx + i
+                    if 0 <= r < lq and 0 <= c < lt and mxs[tx] < mm[r][c]:
+                        mxs[tx] = f32(mm[r][c])
+                        mxc[tx] = (c << 16) | r
+            ndx = D - 1
+            xtl = x0 + D - 1
+            if xtl >= lt:
+                ndx = 0
+            elif xtl + ndx >= lt:
+                ndx = lt - xtl - 1
+            for tx in range(D):
+                w = x0 + tx + ndx
+                if 0 <= w < lt:
+                    writes.append((w, D <= y, mxs[tx], mxc[tx]))
+        # the blocks of one launch touch disjoint buffer columns: all reads
+        # (above) precede all writes within a block, and blocks do not
+        # share columns
+        for w, merge, v, cxy in writes:
+            if not merge or diagm[w] < v:
+                diagm[w] = v
+                mco[w] = cxy
+    ms = [f32(0)] * 64
+    for tx in range(32):
+        if tx < lt:
+            ms[tx] = diagm[lt - 1 - tx]
+        if tx + 32 < lt:
+            ms[tx + 32] = diagm[lt - 1 - tx - 32]
+    nd_ = [0] * 64
+    for tx in range(32):
+        if ms[tx] < ms[tx + 32]:
+            ms[tx] = ms[tx + 32]
+            nd_[tx] = tx + 32
+        else:
+            nd_[tx] = tx
+    i = 16
+    while i >= 1:
+        old_ms, old_nd = list(ms), list(nd_)
+        for tx in range(32):
+            if old_ms[tx] < old_ms[tx + i]:
+                ms[tx] = old_ms[tx + i]
+                nd_[tx] = old_nd[tx + i]
+        i >>= 1
+    return ms[0], int(mco[lt - 1 - nd_[0]])
