@@ -31,6 +31,7 @@
 #include <hip/hip_runtime.h>
 
 #include "cth_batch.h"
+#include "cth_flow.h"
 #include "cth_dev.h"
 #include "cth_dp.h"
 
@@ -53,17 +54,6 @@ __device__ __forceinline__ int corner_maxlen(int lq, int lt) {
 __device__ __forceinline__ bool in_valid_area(int lq, int lt, int edge, int y, int x) {
   const bool corners = (lq <= y + edge && x <= edge + y - lq) || (lt - edge + y <= x && y < edge);
   return !corners && 0 <= y && y < lq && 0 <= x && x < lt;
-}
-
-// MaxIterationsForBlock: the columns the reference's block starting at x0
-// in the block row ending at query row y iterates over
-__device__ __forceinline__ int block_iters(int x0, int y, int lq, int lt) {
-  int x = lt - x0;
-  if (x <= 0) return 0;
-  const int yy = y - (lq - 1);
-  if (0 < yy) x -= yy;
-  if (x <= 0) return 0;
-  return x < 32 ? x : 32;
 }
 
 // The DP of query qp against template profile p, whose scores begin at
@@ -254,12 +244,8 @@ __device__ __forceinline__ void dp_pair(const cth_dpset& q, int qp, const cth_dp
       if ((t & 31) == 30) {
         // a block ends: write the buffer, the lower half-wavefront first
         const int x0 = t - 94 - hi;
-        int ndx = 31;
-        const int xtl = x0 + 31;
-        if (lt <= xtl) ndx = 0;
-        else if (lt <= xtl + ndx) ndx = lt - xtl - 1;
-        const int w = x0 + tx + ndx;
-        const bool wr = bandok && -32 <= x0 && x0 < lt && block_iters(x0, yb, lq, lt) >= 1 && 0 <= w && w < lt;
+        const int w = flow_write_column(x0, tx, lt);
+        const bool wr = bandok && -32 <= x0 && x0 < lt && flow_block_iters(x0, yb, lq, lt) >= 1 && 0 <= w && w < lt;
         if (wr && hi == 0 && (band == 0 || bmv[w] < mv)) {
           bmv[w] = mv;
           bmc[w] = mc;
@@ -274,23 +260,8 @@ __device__ __forceinline__ void dp_pair(const cth_dpset& q, int qp, const cth_dp
     }
     __syncthreads();
   }
-  // FinalizeDPPro: column lt-1-l against lt-33-l, then halves of 32 .. 1, a
-  // tie keeping the lower index (the later column)
-  float v = lane < lt ? bmv[lt - 1 - lane] : 0.0f;
-  int ix = lane;
-  {
-    const float o = __shfl_down(v, 32);
-    if (lane < 32 && v < o) { v = o; ix = lane + 32; }
-  }
-  for (int h = 16; h >= 1; h >>= 1) {
-    const float o = __shfl_down(v, h);
-    const int oi = __shfl_down(ix, h);
-    if (lane < 32 && v < o) { v = o; ix = oi; }
-  }
-  const float vb = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
-  const int ib = __builtin_amdgcn_readfirstlane(ix);
-  best_out = vb;
-  bestxy_out = bmc[lt - 1 - (ib < lt ? ib : lt - 1)];
+  // FinalizeDPPro's reduction of the buffer's last 64 columns
+  flow_final(bmv, bmc, lt, best_out, bestxy_out);
 }
 
 template <bool kGlobalRow>

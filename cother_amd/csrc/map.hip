@@ -31,6 +31,7 @@
 
 #include "cth_batch.h"
 #include "cth_dev.h"
+#include "cth_flow.h"
 #include "cth_map.h"
 #include "cth_stats.h"
 #include "cth_math.h"
@@ -305,8 +306,17 @@ __global__ __launch_bounds__(64) void k_map_bwd(cth_dpset q, int qp, cth_dpset d
   if (threadIdx.x == 0) res[blockIdx.x].logz = logz;
 }
 
-// The MAP alignment DP over the posteriors (traceback bytes to btck); the
-// first maximum in row-major order and its coordinates reach every lane.
+// The MAP alignment DP over the posteriors (traceback bytes to btck); bot:
+// [3][lt] (LDS or global): the stripe boundary row and the reference's buffer
+// of maxima, values and cells.  The maximum and its cell follow that buffer
+// as in dp_pair (dp.hip), with the MAP kernel's two rules
+// (cumapdp/CuBatchMAPDP_map_btck.cu:232-244,349-370): a block whose first
+// column is at or past the end seeds from zero, and before the write a
+// maximum whose cell lies in the ignored corners is reset to zero at cell 0.
+// The maximum is taken over the match state, as there.  (The reference's
+// threads also compute cells outside the matrix; with posterior 0 each of
+// them adds exp(-logz) - minpp < 0 to its diagonal predecessor whenever
+// logz > log(1/minpp), so none of them can hold a thread's maximum then.)
 __device__ __forceinline__ void map_align_one(const cth_dpset& q, int qp, const cth_dpset& db, int p, int64_t colp,
                                               int64_t ldsc, const float* __restrict__ post,
                                               uint8_t* __restrict__ btck, float minpp, float logz, float* bot,
@@ -315,18 +325,40 @@ __device__ __forceinline__ void map_align_one(const cth_dpset& q, int qp, const 
   const int lq = q.len[qp], lt = db.len[p];
   const int edge = corner_maxlen(lq, lt);
   const float gap = minpp * 0.5f;
-  float best = 0.0f;
-  uint32_t bestxy = 0;
+  float* const bmv = bot + lt;
+  uint32_t* const bmc = reinterpret_cast<uint32_t*>(bot + 2 * lt);
   const int nstripes = (lq + 63) >> 6;
   const int c0 = -1 - (lane & 31);
+  const int hi = lane & 32;           // 32 for the upper half-wavefront
+  const int tx = 31 - (lane & 31);    // the reference's thread in its block
   for (int s = 0; s < nstripes; s++) {
     const int y = (s << 6) + lane;
     const bool rowok = y < lq;
+    const int band = 2 * s + (hi >> 5);
+    const int yb = 32 * band + 31;
+    const bool bandok = 32 * band < lq;
+    float mv = 0.0f;
+    uint32_t mc = 0;
     float cA = 0.0f, pA = 0.0f;
-    const int nsteps = lt + 95;
+    // the last block of the upper half ends at step 32 ceil(lt/32) + 94
+    const int nsteps = 32 * ((lt + 31) >> 5) + 95;
     for (int t = 0; t < nsteps; t++) {
       const int c = t - lane - 32;
       const int cu = t - 32;
+      if ((t & 31) == 31) {
+        // a block starts: seed from the buffer at the column before
+        const int x0 = t - 63 - hi;
+        if (bandok && -32 <= x0 && x0 < lt) {
+          const int x = x0 + tx;
+          if (0 <= x - 1 && x < lt) {
+            mv = bmv[x - 1];
+            mc = bmc[x - 1];
+          } else {
+            mv = 0.0f;
+            mc = 0u;
+          }
+        }
+      }
       const float b = (s > 0 && 0 <= cu && cu < lt) ? bot[cu] : 0.0f;
       float uA = dpp_shr1(b, cA);
       if ((lane & 31) == 0 && c < 0) uA = 0.0f;
@@ -338,7 +370,7 @@ __device__ __forceinline__ void map_align_one(const cth_dpset& q, int qp, const 
         float v = pA + cth_fastexpf(P - logz) - minpp;
         int bk = CTH_BT_DIAG;
         if (!(v > 0.0f)) { v = 0.0f; bk = CTH_BT_STOP; }
-        if (valid && best < v) { best = v; bestxy = ((uint32_t)c << 16) | (uint32_t)y; }
+        if (0 <= c && mv < v) { mv = v; mc = ((uint32_t)c << 16) | (uint32_t)y; }
         if (v < uA - gap) { v = uA - gap; bk = CTH_BT_UP; }
         if (v < cA - gap) { v = cA - gap; bk = CTH_BT_LEFT; }
         cA = v;
@@ -346,18 +378,31 @@ __device__ __forceinline__ void map_align_one(const cth_dpset& q, int qp, const 
       }
       pA = uA;
       if (lane == 63 && 0 <= c && c < lt) bot[c] = cA;
+      if ((t & 31) == 30) {
+        // a block ends: write the buffer, the lower half-wavefront first
+        const int x0 = t - 94 - hi;
+        const int w = flow_write_column(x0, tx, lt);
+        const bool wr = bandok && -32 <= x0 && x0 < lt && flow_block_iters(x0, yb, lq, lt) >= 1 && 0 <= w && w < lt;
+        if (wr && !in_valid_area(lq, lt, edge, (int)(mc & 0xffff), (int)(mc >> 16))) {
+          mv = 0.0f;
+          mc = 0u;
+        }
+        if (wr && hi == 0 && (band == 0 || bmv[w] < mv)) {
+          bmv[w] = mv;
+          bmc[w] = mc;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        if (wr && hi != 0 && bmv[w] < mv) {
+          bmv[w] = mv;
+          bmc[w] = mc;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+      }
     }
     __syncthreads();
   }
-  for (int off = 32; off >= 1; off >>= 1) {
-    const float ob = __shfl_xor(best, off);
-    const uint32_t oxy = __shfl_xor(bestxy, off);
-    const uint32_t my = bestxy & 0xffff, oy = oxy & 0xffff;
-    const bool take = ob > best || (ob == best && (oy < my || (oy == my && (oxy >> 16) < (bestxy >> 16))));
-    if (take) { best = ob; bestxy = oxy; }
-  }
-  best_out = best;
-  bestxy_out = bestxy;
+  // FinalizeMAPDP_MAP's reduction of the buffer's last 64 columns
+  flow_final(bmv, bmc, lt, best_out, bestxy_out);
 }
 
 template <bool kGlobalRow>
@@ -366,7 +411,8 @@ __global__ __launch_bounds__(64) void k_map_align(cth_dpset q, int qp, cth_dpset
                                                   const float* __restrict__ post, uint8_t* __restrict__ btck,
                                                   float minpp, cth_mapres* __restrict__ res, float* __restrict__ grow,
                                                   int64_t growstride) {
-  // [lt]: bottom row of the previous stripe (LDS, or this wavefront's part of grow)
+  // [3][lt]: bottom row of the previous stripe and the buffer of maxima (LDS,
+  // or this wavefront's part of grow)
   extern __shared__ float lds_row[];
   float* const bot = kGlobalRow ? grow + (int64_t)blockIdx.x * growstride : lds_row;
   const int p = jobs[blockIdx.x].profile;
@@ -553,8 +599,10 @@ extern "C" int cth_launch_map(const cth_dpset* q, int qp, int lq, const cth_dpse
     hipLaunchKernelGGL(k_map_bwd<false>, dim3(njobs), dim3(64), lds5, st, *q, qp, *db, jobs, col0, ldsc, scores,
                        mods, post, zrow, (int64_t)lq, res, nullptr, (int64_t)0);
     CTH_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_map_align<false>, dim3(njobs), dim3(64), sizeof(float) * (size_t)(ltmax > 0 ? ltmax : 1),
-                       st, *q, qp, *db, jobs, col0, ldsc, post, btck, minpp, res, nullptr, (int64_t)0);
+    // the boundary row and the buffer of maxima (values, cells)
+    hipLaunchKernelGGL(k_map_align<false>, dim3(njobs), dim3(64),
+                       sizeof(float) * 3 * (size_t)(ltmax > 0 ? ltmax : 1), st, *q, qp, *db, jobs, col0, ldsc, post,
+                       btck, minpp, res, nullptr, (int64_t)0);
   }
   CTH_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_map_fin, dim3(njobs), dim3(64), 0, st, *q, qp, *db, jobs, col0, ldsc, scores, btck, alnout,

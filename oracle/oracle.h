@@ -79,6 +79,10 @@ void orc_score_matrix(const orc_tables* t, const orc_pset* q, int qp, const orc_
                       float* scores, float* modscores, int nthreads);
 
 /* phase-1 DP + finalisation (orc_dp.c) */
+/* the maximum and its cell (x << 16 | y) of values mm [lq][lt] as the
+ * reference's per-column buffer of maxima yields them (phase 1: edge < 0;
+ * MAP alignment DP: edge = the ignored corners' side) */
+void orc_flow_max(const float* mm, int lq, int lt, int edge, float* maxsc, uint32_t* maxxy);
 void orc_dp_phase1(const orc_pset* q, int qp, const orc_pset* db, int p0, int p1, const float* sc, uint8_t* bt,
                    float scorethld, float* maxsc, uint32_t* maxxy, int* passed, float* score, int* psts,
                    uint32_t* beg, int nthreads);

@@ -24,7 +24,7 @@
  *     corner triangles, CellYXinValidArea, cudp/CuBatchDP_init_btck.cuh:118-170)
  *     is -9 (:241-251).
  * The maximum MM value and its cell follow the reference's flow through its
- * per-column buffer, so that equal maxima resolve as there (ref_max below):
+ * per-column buffer, so that equal maxima resolve as there (orc_flow_max):
  * each thread of a 32x32 block keeps the running maximum of the cells it
  * computes, seeded from the buffer at the column before its first cell and
  * written back at the block's end (:452-480), and FinalizeDPPro
@@ -87,7 +87,7 @@ static int block_iters(int x0, int y, int lq, int lt) {
   return x < 32 ? x : 32;
 }
 
-/* The reference's maximum of the MM values mm [lq][lt] and its cell
+/* The reference's maximum of the values mm [lq][lt] and its cell
  * (x << 16 | y).  A block row (query rows 32b .. 32b+31) runs its blocks
  * x0 = -32, 0, 32, ... left to right; thread tx of a block takes query row
  * 32b+31-tx, columns x0+tx .. x0+tx+iters-1.  At the block's start each
@@ -100,8 +100,14 @@ static int block_iters(int x0, int y, int lq, int lt) {
  * block rows' writes and reads alternate top to bottom, so block rows are
  * replayed one after another.  FinalizeDPPro then reduces the last 64
  * columns pairwise (column lt-1-t against lt-33-t), then over halves of
- * 32 .. 1, a tie keeping the lower index, i.e. the later column. */
-static void ref_max(const float* mm, int lq, int lt, float* maxsc, uint32_t* maxxy) {
+ * 32 .. 1, a tie keeping the lower index, i.e. the later column.
+ * The MAP alignment DP (cumapdp/CuBatchMAPDP_map_btck.cu:232-244,341-370)
+ * differs in two rules, taken when edge >= 0: a thread whose first column
+ * is at or past the end starts from zero, and before the write a maximum
+ * whose cell lies outside the valid area (the ignored corners, edge = their
+ * side) is reset to zero at cell 0.  (Its threads also carry the values of
+ * cells outside the matrix, which are not represented here.) */
+void orc_flow_max(const float* mm, int lq, int lt, int edge, float* maxsc, uint32_t* maxxy) {
   float* buf = (float*)calloc((size_t)lt, sizeof(float));
   uint32_t* bco = (uint32_t*)calloc((size_t)lt, sizeof(uint32_t));
   for (int b = 0; 32 * b < lq; b++) {
@@ -113,8 +119,9 @@ static void ref_max(const float* mm, int lq, int lt, float* maxsc, uint32_t* max
       uint32_t mc[32];
       for (int tx = 0; tx < 32; tx++) {
         const int xp = x0 + tx - 1;
-        ms[tx] = (0 <= xp && xp < lt) ? buf[xp] : 0.0f;
-        mc[tx] = (0 <= xp && xp < lt) ? bco[xp] : 0u;
+        const int in = 0 <= xp && (edge < 0 ? xp < lt : xp + 1 < lt);
+        ms[tx] = in ? buf[xp] : 0.0f;
+        mc[tx] = in ? bco[xp] : 0u;
       }
       for (int tx = 0; tx < 32; tx++) {
         const int r = y - tx;
@@ -133,6 +140,10 @@ static void ref_max(const float* mm, int lq, int lt, float* maxsc, uint32_t* max
       for (int tx = 0; tx < 32; tx++) {
         const int w = x0 + tx + ndx;
         if (w < 0 || lt <= w) continue;
+        if (0 <= edge && !in_valid_area(lq, lt, edge, (int)(mc[tx] & 0xffff), (int)(mc[tx] >> 16))) {
+          ms[tx] = 0.0f;
+          mc[tx] = 0u;
+        }
         if (b == 0 || buf[w] < ms[tx]) { buf[w] = ms[tx]; bco[w] = mc[tx]; }
       }
     }
@@ -227,7 +238,7 @@ static void dp_profile(const orc_pset* q, int qp, const orc_pset* db, int tp, co
     }
     float* t = prev; prev = cur; cur = t;
   }
-  ref_max(mmv, lq, lt, maxsc, maxxy);
+  orc_flow_max(mmv, lq, lt, -1, maxsc, maxxy);
   free(mmv); free(prev); free(cur); free(dtr);
 }
 

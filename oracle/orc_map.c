@@ -264,7 +264,8 @@ static int in_valid_area(int lq, int lt, int edge, int y, int x) {
   return !corners && 0 <= y && y < lq && 0 <= x && x < lt;
 }
 
-/* MAP alignment DP: bt [lq][lt] traceback bytes, *maxsc / *maxxy (x<<16|y) */
+/* MAP alignment DP: bt [lq][lt] traceback bytes, *maxsc / *maxxy (x<<16|y)
+ * as the reference's buffer of maxima yields them (orc_flow_max) */
 void orc_map_align(int lq, int lt, const float* post, float logz, float minpp, uint8_t* bt, float* maxsc,
                    uint32_t* maxxy) {
   const int edge = corner_maxlen(lq, lt);
@@ -272,8 +273,7 @@ void orc_map_align(int lq, int lt, const float* post, float logz, float minpp, u
   const int W = lt + 33;
   float* prev = (float*)calloc((size_t)W, sizeof(float));
   float* cur = (float*)calloc((size_t)W, sizeof(float));
-  float best = 0.0f;
-  uint32_t bestxy = 0;
+  float* vm = (float*)calloc((size_t)lq * (size_t)lt, sizeof(float));
   for (int y = 0; y < lq; y++) {
     const int c0 = -1 - (y % 32);
     if (y % 32 == 0) memset(prev, 0, sizeof(float) * 33);
@@ -288,7 +288,7 @@ void orc_map_align(int lq, int lt, const float* post, float logz, float minpp, u
       float v = dg + orc_fastexpf(p - logz) - minpp;
       int b = BT_DIAG;
       if (!(v > 0.0f)) { v = 0.0f; b = BT_STOP; }
-      if (valid && best < v) { best = v; bestxy = ((uint32_t)c << 16) | (uint32_t)y; }
+      if (c >= 0) vm[(int64_t)y * lt + c] = v;
       if (v < up - gap) { v = up - gap; b = BT_UP; }
       if (v < lf - gap) { v = lf - gap; b = BT_LEFT; }
       cur[c + 33] = v;
@@ -296,9 +296,8 @@ void orc_map_align(int lq, int lt, const float* post, float logz, float minpp, u
     }
     float* t = prev; prev = cur; cur = t;
   }
-  free(prev); free(cur);
-  *maxsc = best;
-  *maxxy = bestxy;
+  orc_flow_max(vm, lq, lt, edge, maxsc, maxxy);
+  free(vm); free(prev); free(cur);
 }
 
 /* FinalizeMAPDP_MAP traceback (cumapdp/CuBatchMAPDP_map_final.cu:210-336):

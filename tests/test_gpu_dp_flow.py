@@ -68,3 +68,26 @@ def test_random_short_templates(ctx, oracle, seed):
         assert (bt == obt).all()
         for f in ("maxsc", "maxxy", "passed", "psts", "beg"):
             assert (res[f] == ores[f]).all(), (qi, f)
+
+
+def test_search_realigns_short_templates(ctx, oracle):
+    """whole searches at a permissive E-value, so that MAP realigns many hits
+    against templates of 6-40 positions (the MAP alignment DP's buffer drops
+    rows there too): hit lists and MAP coordinates as the oracle's"""
+    q = cother_amd.synth(2, 0, lmin=40, lmax=120, dist_mode=0, sort_desc=0, seed=61)
+    db = cother_amd.concat([cother_amd.subset(q, 0, 1),
+                            cother_amd.synth(200, 0, lmin=6, lmax=40, dist_mode=1, sort_desc=1, seed=62)])
+    ctx.upload_queries(q)
+    ctx.upload_db(db)
+    hits, nh, _ = ctx.search(eval=1e4)
+    off = np.concatenate([[0], np.cumsum(nh)])
+    realigned = 0
+    for qi in range(q.nprof):
+        want = oracle.search_query(q, qi, db, eval_=1e4)
+        got = hits[off[qi]:off[qi + 1]]
+        assert len(got) == len(want)
+        for f in ("profile", "score", "beg", "end", "mapbeg", "mapend", "alnlen", "idnts", "ngaps"):
+            assert (got[f] == want[f]).all(), (qi, f)
+        realigned += int((got["alnlen"] > 0).sum())
+    print(f"{len(hits)} hits, {realigned} realigned", flush=True)
+    assert realigned > 20
