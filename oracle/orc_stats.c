@@ -34,6 +34,13 @@
 
 #include "oracle.h"
 
+/* the continued fractions' underflow guard and tolerance (the single-precision
+ * twin, orc_stats32.c, sets its own) */
+#ifndef ORC_FPMIN
+#define ORC_FPMIN 1e-300
+#define ORC_EPS 1e-15
+#endif
+
 #define NSC 21
 #define SCALE 32
 #define SSCMIN (-320)
@@ -257,7 +264,7 @@ void orc_stat_params(const double counts[NSSC], double out[6]) {
 /* ---- special functions ------------------------------------------------------ */
 /* continued fraction for the incomplete beta (Lentz) */
 static double betacf(double a, double b, double x) {
-  const double FPMIN = 1e-300, EPS = 1e-15;
+  const double FPMIN = ORC_FPMIN, EPS = ORC_EPS;
   const double qab = a + b, qap = a + 1.0, qam = a - 1.0;
   double c = 1.0, d = 1.0 - qab * x / qap;
   if (fabs(d) < FPMIN) d = FPMIN;
@@ -306,7 +313,7 @@ double orc_lgammaincq(double a, double x) {
     const double P = sum * exp(-x + a * log(x) - gln);
     return log1p(-P);
   }
-  const double FPMIN = 1e-300;
+  const double FPMIN = ORC_FPMIN;
   double b = x + 1.0 - a, c = 1.0 / FPMIN, d = 1.0 / b, h = d;
   for (int i = 1; i < 100000; i++) {
     const double an = -i * (i - a);
