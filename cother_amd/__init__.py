@@ -121,6 +121,8 @@ def lib():
                              ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(SearchStats)]
     L.cth_search_noalloc.argtypes = L.cth_search.argtypes
     L.cth_search_batch.argtypes = L.cth_search.argtypes
+    L.cth_search_batch_list.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(SearchOpts),
+                                        ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(SearchStats)]
     L.cth_batch_graph_counts.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64),
                                          ctypes.POINTER(ctypes.c_int64)]
     VP = ctypes.POINTER(ctypes.c_void_p)
@@ -538,27 +540,57 @@ class Context:
             alns.append(tuple(raw[o + r * n:o + (r + 1) * n].decode() for r in range(5)))
         return hits, nh, st, alns
 
+    def search_list(self, queries, eval=10.0, ssemodel=2, do_map=True, timing=False, mem_budget=0, minpp=MINPP,
+                    alignments=False):
+        """cth_search_batch_list: the batched search for the distinct query indices `queries`, in that
+        order; returns (hits grouped by query in list order, per-query hit counts, SearchStats) and with
+        alignments=True the five-row alignments per hit."""
+        ql = np.ascontiguousarray(queries, np.int32)
+        maxhits = max(1, len(ql) * self.db.nprof)
+        hits = np.zeros(maxhits, HIT)
+        nh = np.zeros(len(ql), np.int32)
+        st = SearchStats()
+        abuf = None
+        if alignments:
+            cap = 5 * min(maxhits, 4096) * int(self.qs.lens()[ql].max() + self.db.lens().max())
+            abuf = ctypes.create_string_buffer(cap)
+        opts = SearchOpts(eval, ssemodel, int(do_map), int(timing), mem_budget, minpp, 0,
+                          ctypes.cast(abuf, ctypes.c_void_p) if abuf is not None else None,
+                          len(abuf) if abuf is not None else 0, 0, 0)
+        _check(lib().cth_search_batch_list(self.h, ql.ctypes.data, len(ql), ctypes.byref(opts), hits.ctypes.data,
+                                           maxhits, nh.ctypes.data, ctypes.byref(st)), "cth_search_batch_list")
+        hits = hits[:int(nh.sum())]
+        if not alignments:
+            return hits, nh, st
+        raw = abuf.raw
+        alns = [tuple(raw[int(h["alnoff"]) + r * int(h["alnlen"]):int(h["alnoff"]) + (r + 1) * int(h["alnlen"])]
+                      .decode() for r in range(5)) for h in hits]
+        return hits, nh, st, alns
+
+
 class ServerOpts(ctypes.Structure):
     """Mirror of cth_server_opts (include/cother_hip.h)."""
     _fields_ = [("nslots", ctypes.c_int), ("mem_budget", ctypes.c_uint64), ("chunk_cols", ctypes.c_uint64),
-                ("search", SearchOpts), ("alncap", ctypes.c_uint64)]
+                ("search", SearchOpts), ("alncap", ctypes.c_uint64), ("batch", ctypes.c_int)]
 
 
 class RequestInfo(ctypes.Structure):
     """Mirror of cth_request_info (include/cother_hip.h)."""
     _fields_ = [("qidx", ctypes.c_int32), ("slot", ctypes.c_int32), ("footprint", ctypes.c_uint64),
                 ("ms_queued", ctypes.c_double), ("ms_latency", ctypes.c_double), ("ms_service", ctypes.c_double),
-                ("stats", SearchStats)]
+                ("stats", SearchStats), ("group", ctypes.c_int32)]
 
 
 class Server:
     """The serving loop (cth_server_*) over a context's uploaded queries and resident database."""
 
     def __init__(self, ctx, nslots=2, mem_budget=64 << 30, chunk_cols=4 << 20, eval=10.0, ssemodel=2, do_map=True,
-                 minpp=MINPP, alncap=0, contraction=0):
+                 minpp=MINPP, alncap=0, contraction=0, batch=1):
+        """batch: requests a slot may take in one tick (run together by cth_search_batch_list)."""
         o = ServerOpts()
         lib().cth_server_opts_init(ctypes.byref(o))
         o.nslots, o.mem_budget, o.chunk_cols, o.alncap = nslots, mem_budget, chunk_cols, alncap
+        o.batch = batch
         o.search.eval, o.search.ssemodel, o.search.do_map, o.search.minpp = eval, ssemodel, int(do_map), minpp
         o.search.contraction = contraction
         self.ctx = ctx

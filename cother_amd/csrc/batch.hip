@@ -185,7 +185,7 @@ struct batch_layout {
 
 // One batch's description on the host.
 struct batch_plan {
-  int q0, q1;                       // queries [q0,q1)
+  int q0, q1;                       // entries [q0,q1) of the query list
   std::vector<cth_bquery> bq;
   std::vector<cth_bytile> ytab;
   std::vector<cth_bpair> pairs;
@@ -262,11 +262,12 @@ int capture_or_launch(cth_ctx* c, const batch_plan& P, const batch_layout& L, co
 
 }  // namespace
 
-extern "C" int cth_search_batch(cth_ctx* c, int q0, int q1, const cth_search_opts* opt, cth_hit* hits, int maxhits,
-                                int* nhits, cth_search_stats* stats) {
-  if (!c->has_q || !c->has_db) { cth_set_error("search_batch: queries or database not uploaded"); return -1; }
-  if (q0 < 0 || q1 > c->q.d.nprof || q0 >= q1) { cth_set_error("search_batch: bad query range [%d,%d)", q0, q1); return -1; }
-  if (cth_search_opts_check(opt, "search_batch")) return -1;
+namespace {
+
+// the batched search over the queries ql[0..nq) (distinct indices), hits per
+// query in list order; nhits[k] for query ql[k]
+int search_batch_list(cth_ctx* c, const int* ql, int nq, const cth_search_opts* opt, cth_hit* hits, int maxhits,
+                      int* nhits, cth_search_stats* stats) {
   CTH_HIP_CHECK(hipSetDevice(c->device));
   const cth_devset& db = c->db;
   const int nprof = db.d.nprof;
@@ -276,7 +277,28 @@ extern "C" int cth_search_batch(cth_ctx* c, int q0, int q1, const cth_search_opt
   // the batched form covers the FP32 contraction and profiles within the LDS
   // row limit; otherwise the queries run through cth_search
   if (opt->contraction != 0 || ltmax > CTH_DP_LDS_ROW_MAX || nprof == 0) {
-    if (cth_search(c, q0, q1, opt, hits, maxhits, nhits, stats)) return -1;
+    int nh = 0;
+    uint64_t used = 0;
+    for (int k = 0; k < nq; k++) {
+      cth_search_opts o1 = *opt;
+      if (opt->alnbuf) {
+        o1.alnbuf = opt->alnbuf + used;
+        o1.alncap = opt->alncap - used;
+      }
+      int n1 = 0;
+      cth_search_stats s1{};
+      if (cth_search(c, ql[k], ql[k] + 1, &o1, hits + nh, maxhits - nh, &n1, &s1)) return -1;
+      for (int j = 0; j < n1; j++) {
+        if (opt->alnbuf) hits[nh + j].alnoff += used;
+        if (opt->alnbuf && opt->do_map) used += 5 * (uint64_t)hits[nh + j].alnlen;
+      }
+      if (nhits) nhits[k] = n1;
+      nh += n1;
+      st.cells += s1.cells; st.npassed += s1.npassed; st.nchunks += s1.nchunks; st.map_cells += s1.map_cells;
+      st.ms_phase1 += s1.ms_phase1; st.ms_phase2 += s1.ms_phase2; st.ms_k_score += s1.ms_k_score;
+      st.ms_k_dp += s1.ms_k_dp; st.ms_k_stats += s1.ms_k_stats; st.ms_k_map += s1.ms_k_map;
+    }
+    if (stats) *stats = st;
     return 0;
   }
   const uint64_t dbsize = opt->dbsize ? opt->dbsize : (uint64_t)npos;
@@ -303,15 +325,18 @@ extern "C" int cth_search_batch(cth_ctx* c, int q0, int q1, const cth_search_opt
 
   int nh = 0;
   uint64_t alnused = 0;
-  for (int qa = q0; qa < q1;) {
-    // the batch: consecutive queries whose buffers fit the budget together
+  // batch-local index of each query of the current batch (for its hits)
+  std::vector<int> local(c->q.d.nprof, -1);
+  for (int qa = 0; qa < nq;) {
+    // the batch: consecutive entries of the list whose buffers fit the
+    // budget together
     batch_plan P;
     P.q0 = qa;
     size_t bytes = 0;
     int qb = qa;
     int tiles = 0;
-    while (qb < q1) {
-      const int lq = c->q.h_len[qb];
+    while (qb < nq) {
+      const int lq = c->q.h_len[ql[qb]];
       const size_t qbytes = (size_t)lq * (size_t)ldsc * bytes_per_cell;
       const int qt = (lq + rows - 1) / rows;
       if (qb > qa && (bytes + qbytes > budget || tiles + qt > 65535)) break;
@@ -330,12 +355,12 @@ extern "C" int cth_search_batch(cth_ctx* c, int q0, int q1, const cth_search_opt
       }
       int n1 = 0;
       cth_search_stats s1{};
-      if (cth_search(c, qa, qa + 1, &o1, hits + nh, maxhits - nh, &n1, &s1)) return -1;
+      if (cth_search(c, ql[qa], ql[qa] + 1, &o1, hits + nh, maxhits - nh, &n1, &s1)) return -1;
       for (int k = 0; k < n1; k++) {
         if (opt->alnbuf) hits[nh + k].alnoff += alnused;
         if (opt->alnbuf && opt->do_map) alnused += 5 * (uint64_t)hits[nh + k].alnlen;
       }
-      if (nhits) nhits[qa - q0] = n1;
+      if (nhits) nhits[qa] = n1;
       nh += n1;
       st.cells += s1.cells; st.npassed += s1.npassed; st.nchunks += s1.nchunks; st.map_cells += s1.map_cells;
       st.ms_phase1 += s1.ms_phase1; st.ms_phase2 += s1.ms_phase2; st.ms_k_score += s1.ms_k_score;
@@ -343,7 +368,9 @@ extern "C" int cth_search_batch(cth_ctx* c, int q0, int q1, const cth_search_opt
       qa++;
       continue;
     }
-    for (int qi = P.q0; qi < P.q1; qi++) {
+    for (int k = P.q0; k < P.q1; k++) {
+      const int qi = ql[k];
+      local[qi] = k - P.q0;
       cth_bquery Q{};
       Q.qp = qi;
       Q.lq = c->q.h_len[qi];
@@ -406,15 +433,25 @@ extern "C" int cth_search_batch(cth_ctx* c, int q0, int q1, const cth_search_opt
                                  c->st));
     CTH_HIP_CHECK(hipMemcpyAsync(L.pairs, P.pairs.data(), sizeof(cth_bpair) * npairs, hipMemcpyHostToDevice, c->st));
     const double t0 = now_ms();
-    // persistent grid sizes: tuned once per shape (database size, pair count)
-    const uint64_t tkey = ((uint64_t)npos << 24) ^ (uint64_t)npairs ^ ((uint64_t)P.lqmax << 52);
+    // persistent grid sizes: tuned once per class of shapes (database size
+    // and pair count rounded up to a power of two), so that batches of
+    // similar size (a serving loop's ticks) share the tuning instead of each
+    // paying for its candidates; shared contexts use their parent's table
+    uint64_t pbucket = 1;
+    while (pbucket < (uint64_t)npairs) pbucket <<= 1;
+    const uint64_t tkey = ((uint64_t)npos << 24) ^ pbucket;
+    cth_ctx* tc = c->borrowed ? c->parent : c;
     launch_shape S{ncu * 20, ncu * 8};
-    const cth_btune* tuned = nullptr;
-    for (int k = 0; k < c->nbtune; k++)
-      if (c->btune[k].key == tkey) tuned = &c->btune[k];
-    if (tuned) {
-      S = launch_shape{tuned->dp_grid, tuned->small_grid};
-    } else {
+    bool tuned = false;
+    {
+      std::lock_guard<std::mutex> g(tc->tune_mu);
+      for (int k = 0; k < tc->nbtune && k < 32; k++)
+        if (tc->btune[k].key == tkey) {
+          S = launch_shape{tc->btune[k].dp_grid, tc->btune[k].small_grid};
+          tuned = true;
+        }
+    }
+    if (!tuned) {
       const launch_shape cand[] = {{ncu * 8, ncu * 4}, {ncu * 20, ncu * 8}, {ncu * 40, ncu * 16}};
       hipEvent_t e0, e1;
       CTH_HIP_CHECK(hipEventCreate(&e0));
@@ -431,9 +468,15 @@ extern "C" int cth_search_batch(cth_ctx* c, int q0, int q1, const cth_search_opt
       }
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);
-      // the table keeps the last 32 shapes
-      c->btune[c->nbtune % 32] = cth_btune{tkey, S.dp_grid, S.small_grid, best};
-      if (c->nbtune < 32) c->nbtune++;
+      // the table keeps the last 32 classes (another slot may have added
+      // this one meanwhile: then its entry stays)
+      std::lock_guard<std::mutex> g(tc->tune_mu);
+      bool have = false;
+      for (int k = 0; k < tc->nbtune && k < 32; k++) have |= tc->btune[k].key == tkey;
+      if (!have) {
+        tc->btune[tc->nbtune % 32] = cth_btune{tkey, S.dp_grid, S.small_grid, best};
+        tc->nbtune++;
+      }
     }
     // the graph: replayed when everything its launches captured is the same
     // (buffers and their layout, sizes, options, the uploaded sets)
@@ -488,7 +531,7 @@ extern "C" int cth_search_batch(cth_ctx* c, int q0, int q1, const cth_search_opt
     for (const cth_hit& h : bh) {
       if (opt->do_map) st.map_cells += (double)c->q.h_len[h.query] * (double)db.h_len[h.profile];
       if (opt->do_map && h.alnlen < 1) continue;
-      per[h.query - P.q0].push_back(h);
+      per[local[h.query]].push_back(h);
     }
     for (int b = 0; b < P.q1 - P.q0; b++) {
       auto& v = per[b];
@@ -513,7 +556,7 @@ extern "C" int cth_search_batch(cth_ctx* c, int q0, int q1, const cth_search_opt
         }
         hits[nh++] = h;
       }
-      if (nhits) nhits[P.q0 + b - q0] = (int)v.size();
+      if (nhits) nhits[P.q0 + b] = (int)v.size();
     }
     st.cells += (double)P.cells;
     st.npassed += ctr[kCtrPassed];
@@ -523,6 +566,35 @@ extern "C" int cth_search_batch(cth_ctx* c, int q0, int q1, const cth_search_opt
   }
   if (stats) *stats = st;
   return 0;
+}
+
+}  // namespace
+
+extern "C" int cth_search_batch(cth_ctx* c, int q0, int q1, const cth_search_opts* opt, cth_hit* hits, int maxhits,
+                                int* nhits, cth_search_stats* stats) {
+  if (!c->has_q || !c->has_db) { cth_set_error("search_batch: queries or database not uploaded"); return -1; }
+  if (q0 < 0 || q1 > c->q.d.nprof || q0 >= q1) { cth_set_error("search_batch: bad query range [%d,%d)", q0, q1); return -1; }
+  if (cth_search_opts_check(opt, "search_batch")) return -1;
+  std::vector<int> ql(q1 - q0);
+  std::iota(ql.begin(), ql.end(), q0);
+  return search_batch_list(c, ql.data(), q1 - q0, opt, hits, maxhits, nhits, stats);
+}
+
+extern "C" int cth_search_batch_list(cth_ctx* c, const int32_t* qidx, int nq, const cth_search_opts* opt,
+                                     cth_hit* hits, int maxhits, int* nhits, cth_search_stats* stats) {
+  if (!c->has_q || !c->has_db) { cth_set_error("search_batch_list: queries or database not uploaded"); return -1; }
+  if (!qidx || nq < 1) { cth_set_error("search_batch_list: empty query list"); return -1; }
+  if (cth_search_opts_check(opt, "search_batch_list")) return -1;
+  std::vector<char> seen(c->q.d.nprof, 0);
+  std::vector<int> ql(nq);
+  for (int k = 0; k < nq; k++) {
+    const int q = qidx[k];
+    if (q < 0 || q >= c->q.d.nprof) { cth_set_error("search_batch_list: query %d out of range", q); return -1; }
+    if (seen[q]) { cth_set_error("search_batch_list: query %d listed twice", q); return -1; }
+    seen[q] = 1;
+    ql[k] = q;
+  }
+  return search_batch_list(c, ql.data(), nq, opt, hits, maxhits, nhits, stats);
 }
 
 // Counts of graph captures and replays of the batched search on a context

@@ -59,6 +59,7 @@ extern "C" int cth_ctx_create(int device, const char* envdir, cth_ctx** out) {
   cth_ctx* c = (cth_ctx*)calloc(1, sizeof(cth_ctx));
   if (!c) { cth_set_error("out of host memory"); return -1; }
   new (&c->nshared) std::atomic<int>(0);
+  new (&c->tune_mu) std::mutex();
   c->device = device;
   int rc = cth_tables_load(envdir, &c->htab);
   if (rc) { cth_set_error("cannot load score tables from %s (code %d)", envdir, rc); free(c); return -1; }
@@ -87,6 +88,7 @@ extern "C" int cth_ctx_create_shared(cth_ctx* parent, cth_ctx** out) {
   cth_ctx* c = (cth_ctx*)calloc(1, sizeof(cth_ctx));
   if (!c) { cth_set_error("out of host memory"); return -1; }
   new (&c->nshared) std::atomic<int>(0);
+  new (&c->tune_mu) std::mutex();
   c->device = parent->device;
   c->borrowed = 1;
   c->parent = parent;
@@ -137,6 +139,7 @@ extern "C" void cth_ctx_destroy(cth_ctx* c) {
   if (c->bcells) (void)hipFree(c->bcells);
   if (c->bsmall) (void)hipFree(c->bsmall);
   if (c->st) (void)hipStreamDestroy(c->st);
+  c->tune_mu.~mutex();
   free(c);
 }
 

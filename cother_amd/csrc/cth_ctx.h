@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <mutex>
 
 #include "cth_dev.h"
 #include "cth_devdb.h"
@@ -41,7 +42,11 @@ struct cth_ctx {
   void* bsmall; size_t cap_bsmall;
   hipGraphExec_t bgraph; int has_bgraph; uint64_t bkey[12];
   int64_t nbgraph_captures, nbgraph_replays;
+  // tuned grid sizes; a shared context reads and adds to its parent's
+  // table (under the parent's tune_mu), so the slots of a serving loop tune
+  // each class of shapes once between them
   cth_btune btune[32]; int nbtune;
+  std::mutex tune_mu;
   // a context made by cth_ctx_create_shared reads its parent's tables,
   // queries and database and frees only its own stream and scratch
   int borrowed;

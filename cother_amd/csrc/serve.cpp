@@ -16,6 +16,11 @@
 //    queue is scanned in arrival order and the first request that fits is
 //    admitted, so a long query that does not fit yet does not hold back
 //    shorter ones behind it;
+//  - ticks: with batch > 1 a free slot takes, besides the first request
+//    that fits, further waiting requests of distinct queries that fit next
+//    to it (up to batch of them), and runs them as one batched search
+//    (cth_search_batch_list: each stage once over all their pairs), so
+//    queries of mixed lengths that arrive together share launches;
 //  - a slot keeps its scratch between requests (freeing device memory
 //    synchronizes the device, which would stall the other slots), except
 //    that scratch beyond both its share of the budget (mem_budget / nslots)
@@ -59,6 +64,7 @@ struct Request {
   uint64_t footprint = 0;
   double t_submit = 0, t_start = 0, t_end = 0;
   int slot = -1;
+  int group = 1;
   bool done = false;
   int rc = 0;
   std::string err;
@@ -94,51 +100,116 @@ struct cth_server {
   }
   uint64_t footprint(int qidx) const { return (uint64_t)qlens[qidx] * chunk_cols() * bytes_per_cell(); }
 
-  // first waiting request that fits (caller holds mu)
-  Request* admissible() {
-    for (auto it = pending.begin(); it != pending.end(); ++it)
-      if (inflight + (*it)->footprint <= opt.mem_budget) {
-        Request* r = *it;
-        pending.erase(it);
-        return r;
-      }
+  // first waiting request that fits, and whose query is not in the tick's
+  // group yet (caller holds mu)
+  Request* admissible(const std::vector<Request*>& group) {
+    for (auto it = pending.begin(); it != pending.end(); ++it) {
+      if (inflight + (*it)->footprint > opt.mem_budget) continue;
+      bool dup = false;
+      for (const Request* g : group) dup |= g->qidx == (*it)->qidx;
+      if (dup) continue;
+      Request* r = *it;
+      pending.erase(it);
+      return r;
+    }
     return nullptr;
+  }
+
+  // one request: cth_search of its query
+  void run_one(cth_ctx* c, Request* r) {
+    cth_search_opts so = opt.search;
+    so.mem_budget = r->footprint;
+    so.alnbuf = nullptr;
+    so.alncap = 0;
+    if (opt.alncap) {
+      r->aln.resize(opt.alncap);
+      so.alnbuf = r->aln.data();
+      so.alncap = opt.alncap;
+    }
+    r->hits.resize(std::max(1, ndb));
+    int nh = 0;
+    r->rc = cth_search(c, r->qidx, r->qidx + 1, &so, r->hits.data(), (int)r->hits.size(), &nh, &r->stats);
+    if (r->rc) r->err = cth_last_error();
+    r->hits.resize(r->rc ? 0 : nh);
+  }
+
+  // a tick's group of requests (distinct queries) in one batched search,
+  // within the sum of their footprints; each request gets its query's hits
+  // and alignment rows, and the tick's statistics
+  void run_group(cth_ctx* c, const std::vector<Request*>& g, uint64_t budget) {
+    const int n = (int)g.size();
+    std::vector<int32_t> ql(n);
+    for (int k = 0; k < n; k++) ql[k] = g[k]->qidx;
+    cth_search_opts so = opt.search;
+    so.mem_budget = budget;
+    std::vector<char> aln(opt.alncap ? (size_t)opt.alncap * n : 0);
+    so.alnbuf = opt.alncap ? aln.data() : nullptr;
+    so.alncap = opt.alncap ? aln.size() : 0;
+    std::vector<cth_hit> hits((size_t)std::max(1, ndb) * n);
+    std::vector<int> nh(n, 0);
+    cth_search_stats st{};
+    const int rc = cth_search_batch_list(c, ql.data(), n, &so, hits.data(), (int)hits.size(), nh.data(), &st);
+    const std::string err = rc ? cth_last_error() : "";
+    size_t at = 0;
+    for (int k = 0; k < n; k++) {
+      Request* r = g[k];
+      r->rc = rc;
+      r->err = err;
+      r->stats = st;
+      if (rc) continue;
+      r->hits.assign(hits.begin() + at, hits.begin() + at + nh[k]);
+      at += nh[k];
+      if (opt.alncap) {
+        // this request's rows, with offsets into its own buffer
+        uint64_t used = 0;
+        for (cth_hit& h : r->hits) used += 5 * (uint64_t)h.alnlen;
+        r->aln.resize(std::max<uint64_t>(used, 1));
+        used = 0;
+        for (cth_hit& h : r->hits) {
+          const uint64_t sz = 5 * (uint64_t)h.alnlen;
+          memcpy(r->aln.data() + used, aln.data() + h.alnoff, sz);
+          h.alnoff = used;
+          used += sz;
+        }
+      }
+    }
   }
 
   void worker(int slot) {
     cth_ctx* c = slots[slot];
+    const size_t maxgroup = opt.batch > 1 ? (size_t)opt.batch : 1;
     for (;;) {
-      Request* r = nullptr;
+      std::vector<Request*> group;
+      uint64_t fp = 0;
       {
         std::unique_lock<std::mutex> lk(mu);
-        cv_work.wait(lk, [&] { return stop || (r = admissible()) != nullptr; });
+        Request* r = nullptr;
+        cv_work.wait(lk, [&] { return stop || (r = admissible(group)) != nullptr; });
         if (!r) return;  // stopping
-        inflight += r->footprint;
+        // the tick: this request and, up to the batch size, the next waiting
+        // ones that fit next to it
+        for (;;) {
+          inflight += r->footprint;
+          fp += r->footprint;
+          r->slot = slot;
+          r->t_start = now_ms();
+          group.push_back(r);
+          if (group.size() >= maxgroup || !(r = admissible(group))) break;
+        }
         busy++;
-        r->slot = slot;
-        r->t_start = now_ms();
       }
-      cth_ctx_trim(c, std::max<uint64_t>(r->footprint, opt.mem_budget / slots.size()));
-      cth_search_opts so = opt.search;
-      so.mem_budget = r->footprint;
-      so.alnbuf = nullptr;
-      so.alncap = 0;
-      if (opt.alncap) {
-        r->aln.resize(opt.alncap);
-        so.alnbuf = r->aln.data();
-        so.alncap = opt.alncap;
-      }
-      r->hits.resize(std::max(1, ndb));
-      int nh = 0;
-      const int rc = cth_search(c, r->qidx, r->qidx + 1, &so, r->hits.data(), (int)r->hits.size(), &nh, &r->stats);
+      cth_ctx_trim(c, std::max<uint64_t>(fp, opt.mem_budget / slots.size()));
+      if (group.size() == 1) run_one(c, group[0]);
+      else run_group(c, group, fp);
       {
         std::lock_guard<std::mutex> lk(mu);
-        r->rc = rc;
-        if (rc) r->err = cth_last_error();
-        r->hits.resize(rc ? 0 : nh);
-        r->t_end = now_ms();
-        r->done = true;
-        inflight -= r->footprint;
+        const double t = now_ms();
+        for (Request* r : group) {
+          r->t_end = t;
+          r->group = (int)group.size();
+          r->done = true;
+        }
+        inflight -= fp;
         busy--;
       }
       cv_done.notify_all();
@@ -240,6 +311,7 @@ extern "C" int cth_server_wait(cth_server* s, uint64_t ticket, cth_hit* hits, in
     info->ms_latency = r->t_end - r->t_submit;
     info->ms_service = r->t_end - r->t_start;
     info->stats = r->stats;
+    info->group = r->group;
   }
   if (r->rc) {
     cth_set_error("%s", r->err.c_str());

@@ -274,13 +274,18 @@ int cth_search_noalloc(cth_ctx* c, int q0, int q1, const cth_search_opts* opt, c
  * graph and replayed while the batch shape stays the same, and the
  * persistent grid sizes are autotuned per batch shape on its first run.
  * Results (hits, their order, alignment rows) equal cth_search's.  Covers
- * the FP32 contraction and databases whose profiles have at most 8192
+ * the FP32 contraction and databases whose profiles have at most 5824
  * positions; other calls, and queries whose buffers alone exceed the budget,
  * run as cth_search runs them.  With timing, the stages run without the
  * graph and their times fill ms_k_score (score), ms_k_dp (both DP passes),
  * ms_k_stats and ms_k_map; nchunks counts batches. */
 int cth_search_batch(cth_ctx* c, int q0, int q1, const cth_search_opts* opt, cth_hit* hits, int maxhits,
                      int* nhits, cth_search_stats* stats);
+/* The same for the nq distinct queries qidx[0..nq) in that order (no
+ * reference counterpart: the serving loop's ticks batch whichever requests
+ * are admitted together); nhits[k] counts query qidx[k]'s hits. */
+int cth_search_batch_list(cth_ctx* c, const int32_t* qidx, int nq, const cth_search_opts* opt, cth_hit* hits,
+                          int maxhits, int* nhits, cth_search_stats* stats);
 /* Graph captures and replays of cth_search_batch on a context. */
 int cth_batch_graph_counts(const cth_ctx* c, int64_t* captures, int64_t* replays);
 
@@ -307,6 +312,7 @@ typedef struct cth_server_opts {
   uint64_t chunk_cols;     /* database columns per chunk of a request (0: the whole database) */
   cth_search_opts search;  /* per-request options (mem_budget, alnbuf, alncap are set per request) */
   uint64_t alncap;         /* per-request alignment-row capacity (0: no rows) */
+  int batch;               /* requests a slot takes in one tick (<= 1: one) */
 } cth_server_opts;
 typedef struct cth_request_info {
   int32_t qidx, slot;
@@ -314,7 +320,8 @@ typedef struct cth_request_info {
   double ms_queued;        /* submit -> start */
   double ms_latency;       /* submit -> done */
   double ms_service;       /* start -> done */
-  cth_search_stats stats;
+  cth_search_stats stats;  /* of the request's tick (the group's batched search, when batched) */
+  int32_t group;           /* requests that ran together in this request's tick */
 } cth_request_info;
 void cth_server_opts_init(cth_server_opts* o);
 /* The parent context holds the uploaded queries and database; create the

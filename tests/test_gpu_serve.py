@@ -56,6 +56,37 @@ def test_results_alone_and_in_the_mix(ctx, sets, nslots, budget_cells):
         srv.close()
 
 
+@pytest.mark.parametrize("nslots,batch", [(1, 4), (2, 8)])
+def test_batched_ticks_alone_and_in_the_mix(ctx, sets, nslots, batch):
+    """with batch > 1 a free slot runs the requests waiting together as one
+    batched search: every request's hits and rows as cth_search alone, and
+    some ticks really hold several requests"""
+    q, db = sets
+    ctx.upload_queries(q)
+    ctx.upload_db(db)
+    want = [_alone(ctx, qi) for qi in range(q.nprof)]
+    # footprints of the whole database (chunk_cols 0): what a batched search
+    # of those queries needs
+    srv = cother_amd.Server(ctx, nslots=nslots, mem_budget=1 << 34, chunk_cols=0, alncap=1 << 22, batch=batch)
+    try:
+        order = list(range(q.nprof)) + [3, 0, 8, 3]  # some queries twice, one three times
+        tickets = [(qi, srv.submit(qi)) for qi in order]
+        groups = []
+        for qi, t in tickets:
+            hits, alns, info = srv.wait(t)
+            assert info["qidx"] == qi
+            groups.append(info["group"])
+            wh, wa = want[qi]
+            assert len(hits) == len(wh), f"query {qi}"
+            for f in ("profile", "score", "logE", "beg", "end", "mapbeg", "mapend", "alnlen", "idnts", "psts"):
+                assert (hits[f].view(np.uint32) == wh[f].view(np.uint32)).all(), f"query {qi} field {f}"
+            assert alns == wa, f"query {qi}"
+        print(f"{len(order)} requests, tick sizes seen {sorted(set(groups))}", flush=True)
+        assert max(groups) > 1 and max(groups) <= batch
+    finally:
+        srv.close()
+
+
 def test_request_over_budget_is_refused(ctx, sets):
     q, db = sets
     ctx.upload_queries(q)
