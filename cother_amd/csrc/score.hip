@@ -46,16 +46,23 @@ __device__ __forceinline__ float ss_lookup(const float* ss, int row, int col) {
   return ss[col + 2];
 }
 
-// A segment's coordinates as packed words: begin B = QB | PB << 16 and end
-// E = QE | PE << 16.  Every index is below 256, so with kEndBias = 0x4000 in
-// each half, (E + kEndBias) - B = 0x4000 + (end - begin) per half with no
-// borrow between the halves, and bit 14 of the half is set exactly when
-// end >= begin.  The register keeps each entry as nb = kEndBias - B and
-// ce = E + kEndBias, so that both differences of a test are one add or
-// subtract from a segment's B and E.
-constexpr uint32_t kEndBias = 0x40004000u;
+// A segment's coordinates as packed words: begin B = QB | PB << 11 and end
+// E = QE | PE << 11.  Every index is below 256, so with kEndBias = 0x200 in
+// each 11-bit half, (E + kEndBias) - B = 0x200 + (end - begin) per half with
+// no borrow between the halves, and bit 9 of the half is set exactly when
+// end >= begin.  Packed words stay below 2^22, so a float 2^23 + W holds W
+// exactly and its bits are kMagic + W: the kernel computes a candidate's
+// begin as one fma on floats (an instruction that dual-issues) and takes its
+// bits, and every word it compares carries the same kMagic, which cancels in
+// each difference of a test.  The register keeps each entry as
+// nb = kEndBias - B and ce = E + kEndBias (with kMagic), so that both
+// differences of a test are one add or subtract from a segment's B and E.
+constexpr uint32_t kPack = 0x801u;        // one index step on both axes
+constexpr float kPackF = 2049.0f;
+constexpr uint32_t kEndBias = 0x00100200u;
+constexpr uint32_t kMagic = 0x4b000000u;  // the bits of 2^23
 // entries that intersect nothing: begins 255, ends 0
-constexpr uint32_t kNoBeginN = kEndBias - 0x00ff00ffu, kNoEnd = kEndBias;
+constexpr uint32_t kNoBeginN = kEndBias - (kMagic + 0x0007f8ffu), kNoEnd = kMagic + kEndBias;
 
 // whether segment s (begin sb, end se) intersects register entry c (nb, ce):
 // on the query or the template axis the byte-wise max of the begins is <=
@@ -153,7 +160,8 @@ __device__ __forceinline__ void register_max(float& m0, float& m1, float& m2, ui
   }
 // one 4-cell step, cells K0+1..K0+4 relative to the base ib, with pair
 // scores A2, A4 (cells K0+2 and K0+4) and the halves H0, H2, H4 of those of
-// cells K0 (the previous step's last cell), K0+2 and K0+4: the reference's
+// cells K0 (the previous step's last cell), K0+2 and K0+4, and FB the float
+// 2^23 + the packed coordinates of cell K0+1 on both axes: the reference's
 // interpolated sum 0.5f * (a + b) is h(a) + h(b), halving being exact and
 // commuting with the rounding of the sum.  The begin and end of the step's
 // best are recovered only when it can be registered.  Three levels: the
@@ -167,7 +175,7 @@ __device__ __forceinline__ void register_max(float& m0, float& m1, float& m2, ui
 // register.  The empty asm statements keep the candidate's packed words in
 // registers: without them the compiler folds their sums into each test as
 // scalar-operand adds, which issue one per cycle.
-#define CTH_DDMS_4CELL(K0, H0, A2, H2, A4, H4, QOFF, POFF)                    \
+#define CTH_DDMS_4CELL(H0, A2, H2, A4, H4, FB)                                \
   {                                                                           \
     float cd1, cd2, cd3, cd4;                                                 \
     CTH_DDMS_CELL(H0 + H2, cd1)                                               \
@@ -177,25 +185,29 @@ __device__ __forceinline__ void register_max(float& m0, float& m1, float& m2, ui
     CTH_DDMS_CELL(A4, cd4)                                                    \
     const float scom = fmaxf(fmaxf(cd1, cd2), fmaxf(cd3, cd4));               \
     if (scom >= m2f) {                                                        \
-      /* the step's first cell on both axes, packed */                        \
-      const uint32_t base = (uint32_t)(ib + (K0) + 1) * 0x10001u +            \
-                            (uint32_t)(QOFF) + ((uint32_t)(POFF) << 16);      \
+      /* the step's first cell on both axes, packed (with kMagic) */          \
+      const uint32_t base = __float_as_uint(FB);                              \
       /* begin: the candidate's segment has DSTSEGM or more cells at a cell   \
          of this step, so it runs at the step's first cell, whose dd gives    \
-         its begin */                                                         \
-      uint32_t sb = base + (uint32_t)__mul24((int)dd1, 0x10001);              \
+         its begin: 2^23 + first cell + dd1 (1 + 2^11), exact in float */     \
+      uint32_t sb = __float_as_uint(fmaf(dd1, packf, FB));                    \
       asm("" : "+v"(sb));                                                     \
       /* the end lies in cells 1..4 and an intersection, once there, stays as \
          the end grows: a segment that meets m0 ending at cell 1 meets it at  \
          its end, and below m0 it cannot change the register */               \
       const bool lt0 = scom < m0;                                             \
       if (!(meets(sb, base, nb0, ce0) && lt0)) {                              \
-        uint32_t kk = 3u * 0x10001u;                                          \
-        kk = cd3 == scom ? 2u * 0x10001u : kk;                                \
-        kk = cd2 == scom ? 0x10001u : kk;                                     \
-        kk = cd1 == scom ? 0u : kk;                                           \
-        /* end (ib + K0 + k) on both axes, packed */                          \
-        uint32_t se = kk + base;                                              \
+        /* end: the first of cells 1..4 reaching scom, the cell k + 1 after   \
+           the first, as 2^23 + first cell + k (1 + 2^11); n_k = 1 where cell \
+           k is below scom and 0 at scom (scom >= cd_k, and sums of the pair  \
+           scores are multiples of 2^-99, so a difference is 0 or scaled by   \
+           2^100 at least 2), so k = n1 (1 + n2 (1 + n3)):                    \
+           fmas that dual-issue, where compares and selects do not */         \
+        const float sbig = scom * big2;                                       \
+        const float n1 = __builtin_amdgcn_fmed3f(fmaf(cd1, -big2, sbig), 0.0f, 1.0f); \
+        const float n2 = __builtin_amdgcn_fmed3f(fmaf(cd2, -big2, sbig), 0.0f, 1.0f); \
+        const float n3 = __builtin_amdgcn_fmed3f(fmaf(cd3, -big2, sbig), 0.0f, 1.0f); \
+        uint32_t se = __float_as_uint(fmaf(n1, fmaf(n2, fmaf(n3, packf, packf), packf), FB)); \
         asm("" : "+v"(se));                                                   \
         const bool i0 = meets(sb, se, nb0, ce0);                              \
         const bool i1 = meets(sb, se, nb1, ce1);                              \
@@ -227,10 +239,12 @@ __device__ __forceinline__ float ddms_cell(const float* __restrict__ dmsT, const
   if (len1 < CTH_DSTSEGM || len2 < CTH_DSTSEGM) return 0.0f;
   float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f, m2f = FLT_MIN;
   uint32_t nb0 = kNoBeginN, ce0 = kNoEnd, nb1 = kNoBeginN, ce1 = kNoEnd;
-  // 2^126 in a vector register: as a scalar operand it would keep the
-  // multiply off the second issue slot
-  float big;
+  // 2^126, 1 + 2^11 and 2^100 in vector registers: as scalar operands they
+  // would keep the multiplies and fmas off the second issue slot
+  float big, packf, big2;
   asm("v_mov_b32 %0, 0x7e800000" : "=v"(big));
+  asm("v_mov_b32 %0, 0x71800000" : "=v"(big2));  // 2^100
+  asm("v_mov_b32 %0, %1" : "=v"(packf) : "i"(__builtin_bit_cast(uint32_t, kPackF)));
   // upper part: diagonal j pairs query index i with template index j + i
   for (int j = len2 >> 2; 0 <= j; j -= 2) {
     const uint8_t* tj = t + 1 + j;
@@ -238,6 +252,8 @@ __device__ __forceinline__ float ddms_cell(const float* __restrict__ dmsT, const
     const float s0 = pair(dmsT, q[1], tj[0]);
     float u = s0 + fabsf(s0);  // 2 max(s0, 0)
     float dd = 0.0f;
+    // 2^23 + the packed coordinates of cell 1 (query 1, template j + 1)
+    float fb = __uint_as_float(kMagic + kPack + ((uint32_t)j << 11));
     float h0 = pairh(dmsT, q[1], tj[0]);
     float s2 = pair(dmsT, q[1 + 2], tj[2]), h2 = pairh(dmsT, q[1 + 2], tj[2]);
     float s4 = pair(dmsT, q[1 + 4], tj[4]), h4 = pairh(dmsT, q[1 + 4], tj[4]);
@@ -246,10 +262,11 @@ __device__ __forceinline__ float ddms_cell(const float* __restrict__ dmsT, const
       // one step ahead (lists are padded)
       const float n2 = pair(dmsT, q[1 + ib + 6], tj[ib + 6]), nh2 = pairh(dmsT, q[1 + ib + 6], tj[ib + 6]);
       const float n4 = pair(dmsT, q[1 + ib + 8], tj[ib + 8]), nh4 = pairh(dmsT, q[1 + ib + 8], tj[ib + 8]);
-      CTH_DDMS_4CELL(0, h0, s2, h2, s4, h4, 0, j)
+      CTH_DDMS_4CELL(h0, s2, h2, s4, h4, fb)
       const float p2 = pair(dmsT, q[1 + ib + 10], tj[ib + 10]), ph2 = pairh(dmsT, q[1 + ib + 10], tj[ib + 10]);
       const float p4 = pair(dmsT, q[1 + ib + 12], tj[ib + 12]), ph4 = pairh(dmsT, q[1 + ib + 12], tj[ib + 12]);
-      if (ib + 4 < iend) CTH_DDMS_4CELL(4, h4, n2, nh2, n4, nh4, 0, j)
+      if (ib + 4 < iend) CTH_DDMS_4CELL(h4, n2, nh2, n4, nh4, fb + 4.0f * kPackF)
+      fb += 8.0f * kPackF;
       h0 = nh4;
       s2 = p2;
       h2 = ph2;
@@ -264,16 +281,19 @@ __device__ __forceinline__ float ddms_cell(const float* __restrict__ dmsT, const
     const float s0 = pair(dmsT, qi[0], t[1]);
     float u = s0 + fabsf(s0);  // 2 max(s0, 0)
     float dd = 0.0f;
+    // 2^23 + the packed coordinates of cell 1 (query i + 1, template 1)
+    float fb = __uint_as_float(kMagic + kPack + (uint32_t)i);
     float h0 = pairh(dmsT, qi[0], t[1]);
     float s2 = pair(dmsT, qi[2], t[1 + 2]), h2 = pairh(dmsT, qi[2], t[1 + 2]);
     float s4 = pair(dmsT, qi[4], t[1 + 4]), h4 = pairh(dmsT, qi[4], t[1 + 4]);
     for (int ib = 0; ib < jend; ib += 8) {
       const float n2 = pair(dmsT, qi[ib + 6], t[1 + ib + 6]), nh2 = pairh(dmsT, qi[ib + 6], t[1 + ib + 6]);
       const float n4 = pair(dmsT, qi[ib + 8], t[1 + ib + 8]), nh4 = pairh(dmsT, qi[ib + 8], t[1 + ib + 8]);
-      CTH_DDMS_4CELL(0, h0, s2, h2, s4, h4, i, 0)
+      CTH_DDMS_4CELL(h0, s2, h2, s4, h4, fb)
       const float p2 = pair(dmsT, qi[ib + 10], t[1 + ib + 10]), ph2 = pairh(dmsT, qi[ib + 10], t[1 + ib + 10]);
       const float p4 = pair(dmsT, qi[ib + 12], t[1 + ib + 12]), ph4 = pairh(dmsT, qi[ib + 12], t[1 + ib + 12]);
-      if (ib + 4 < jend) CTH_DDMS_4CELL(4, h4, n2, nh2, n4, nh4, i, 0)
+      if (ib + 4 < jend) CTH_DDMS_4CELL(h4, n2, nh2, n4, nh4, fb + 4.0f * kPackF)
+      fb += 8.0f * kPackF;
       h0 = nh4;
       s2 = p2;
       h2 = ph2;

@@ -187,5 +187,12 @@ extern "C" int cth_tables_load(const char* envdir, cth_tables* t) {
       }
       t->dms[a * 22 + b] = v;
     }
+  // the score kernel's DDMS scan tells a positive running sum from zero, and
+  // two sums apart, by scaling with 2^126 and 2^100 and clamping to [0, 1]:
+  // exact when every nonzero pair score is at least 2^-75 in magnitude (all
+  // sums of such scores and of their halves are then multiples of 2^-99);
+  // these entries are multiples of 2^-48, the smallest nonzero near 2^-25
+  for (float v : t->dms)
+    if (v != 0.0f && fabsf(v) < 0x1p-75f) return 200;
   return 0;
 }
