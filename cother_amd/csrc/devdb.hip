@@ -55,6 +55,14 @@ extern "C" void cth_devset_free(cth_devset* s) {
 extern "C" int cth_devset_upload(const cth_pset* h, cth_devset* s) {
   memset(s, 0, sizeof *s);
   const int64_t n = h->npos, ld = pad64(n > 0 ? n : 1);
+  // distance lists hold at most MAX_N_IR_DISTANCE_VALUES = 99 values
+  // (libpro/srcpro/MOptions.h:15); the score kernel's scan relies on it
+  for (int64_t i = 0; i < n; i++)
+    if (h->dd[(size_t)i * CTH_DDSTRIDE] > CTH_MAXDST) {
+      cth_set_error("position %lld: distance list of %d values (at most %d)", (long long)i,
+                    (int)h->dd[(size_t)i * CTH_DDSTRIDE], CTH_MAXDST);
+      return -1;
+    }
   const int64_t ntr = h->npos + h->nprof, ldtr = pad64(ntr > 0 ? ntr : 1);
   cth_dpset& d = s->d;
   d.nprof = h->nprof;

@@ -32,8 +32,11 @@ constexpr int kRows = 4;        // query rows per workgroup (one per wavefront)
 constexpr int kCols = 64;       // template columns per workgroup (one per lane)
 constexpr int kListPitch = 132; // LDS bytes per template list
 
+constexpr int kQlEntries = 112;  // query record entries the DDMS scan reads (0..111)
+
 struct ScoreLds {
-  uint8_t tl[kCols * kListPitch];
+  uint8_t tl[kCols * kListPitch];  // template lists, values scaled by 4 (table byte columns)
+  int32_t ql[kRows][kQlEntries];   // each wavefront's query list as table byte offsets
   float dms[22 * 22];
   float dmsh[22 * 22];  // the pair scores halved (exact), right after dms
   float ss[467];
@@ -220,22 +223,27 @@ __device__ __forceinline__ void register_max(float& m0, float& m1, float& m2, ui
     }                                                                         \
   }
 
+// The scan reads everything from the workgroup's LDS block by byte offsets
+// held in vector registers, so that each table address is one vector add
+// (which dual-issues) and the rest is the instruction's immediate offset.
+// kDms, kDmsh: the pair-score table and its halves in the block
+constexpr int kDms = (int)offsetof(ScoreLds, dms), kDmsh = (int)offsetof(ScoreLds, dmsh);
+__device__ __forceinline__ int lds_i32(const char* lds, int off) { return *reinterpret_cast<const int32_t*>(lds + off); }
+__device__ __forceinline__ int lds_u8(const char* lds, int off) { return *reinterpret_cast<const uint8_t*>(lds + off); }
 // the pair score of list values whose table byte offsets are qb (query,
-// d1*88) and tv (template value d2): dmsT[d1][d2]
-__device__ __forceinline__ float pair(const float* __restrict__ dmsT, int qb, int tv) {
-  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(dmsT) + qb + (tv << 2));
+// d1*88) and tv (template, d2*4): dmsT[d1][d2], and its half
+__device__ __forceinline__ float pair(const char* lds, int qb, int tv) {
+  return *reinterpret_cast<const float*>(lds + kDms + qb + tv);
 }
-// the same pair score halved, from the table that follows dmsT (same address
-// plus a constant offset)
-__device__ __forceinline__ float pairh(const float* __restrict__ dmsT, int qb, int tv) {
-  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(dmsT + 22 * 22) + qb + (tv << 2));
+__device__ __forceinline__ float pairh(const char* lds, int qb, int tv) {
+  return *reinterpret_cast<const float*>(lds + kDmsh + qb + tv);
 }
 
-// q: the query list as table byte offsets (entry k = 88 * value k, entry 0 =
-// the list length); t: the template list (byte 0 = length, then values)
-__device__ __forceinline__ float ddms_cell(const float* __restrict__ dmsT, const int32_t* __restrict__ q,
-                           const uint8_t* __restrict__ t) {
-  const int len1 = q[0], len2 = t[0];
+// lds: the workgroup's block; qr: byte offset of the wavefront's query list
+// (int32 table byte offsets, entry 0 = the list length); tr: byte offset of
+// the lane's template list (byte 0 = length, then values times 4)
+__device__ __forceinline__ float ddms_cell(const char* lds, int qr, int tr) {
+  const int len1 = lds_i32(lds, qr), len2 = lds_u8(lds, tr);
   if (len1 < CTH_DSTSEGM || len2 < CTH_DSTSEGM) return 0.0f;
   float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f, m2f = FLT_MIN;
   uint32_t nb0 = kNoBeginN, ce0 = kNoEnd, nb1 = kNoBeginN, ce1 = kNoEnd;
@@ -247,26 +255,37 @@ __device__ __forceinline__ float ddms_cell(const float* __restrict__ dmsT, const
   asm("v_mov_b32 %0, %1" : "=v"(packf) : "i"(__builtin_bit_cast(uint32_t, kPackF)));
   // upper part: diagonal j pairs query index i with template index j + i
   for (int j = len2 >> 2; 0 <= j; j -= 2) {
-    const uint8_t* tj = t + 1 + j;
     const int iend = min(len1, len2 - j);
-    const float s0 = pair(dmsT, q[1], tj[0]);
+    // byte offsets of query entry 1 and template entry j + 1, advanced by
+    // one iteration (8 entries) at a time in vector registers
+    int qo = qr + 4, to = tr + 1 + j;
+    const float s0 = pair(lds, lds_i32(lds, qo), lds_u8(lds, to));
     float u = s0 + fabsf(s0);  // 2 max(s0, 0)
     float dd = 0.0f;
     // 2^23 + the packed coordinates of cell 1 (query 1, template j + 1)
     float fb = __uint_as_float(kMagic + kPack + ((uint32_t)j << 11));
-    float h0 = pairh(dmsT, q[1], tj[0]);
-    float s2 = pair(dmsT, q[1 + 2], tj[2]), h2 = pairh(dmsT, q[1 + 2], tj[2]);
-    float s4 = pair(dmsT, q[1 + 4], tj[4]), h4 = pairh(dmsT, q[1 + 4], tj[4]);
+    float h0 = pairh(lds, lds_i32(lds, qo), lds_u8(lds, to));
+    float s2 = pair(lds, lds_i32(lds, qo + 8), lds_u8(lds, to + 2)),
+          h2 = pairh(lds, lds_i32(lds, qo + 8), lds_u8(lds, to + 2));
+    float s4 = pair(lds, lds_i32(lds, qo + 16), lds_u8(lds, to + 4)),
+          h4 = pairh(lds, lds_i32(lds, qo + 16), lds_u8(lds, to + 4));
     for (int ib = 0; ib < iend; ib += 8) {
       // two steps per iteration; the pair scores of the step after are loaded
       // one step ahead (lists are padded)
-      const float n2 = pair(dmsT, q[1 + ib + 6], tj[ib + 6]), nh2 = pairh(dmsT, q[1 + ib + 6], tj[ib + 6]);
-      const float n4 = pair(dmsT, q[1 + ib + 8], tj[ib + 8]), nh4 = pairh(dmsT, q[1 + ib + 8], tj[ib + 8]);
+      const int q6 = lds_i32(lds, qo + 24), q8 = lds_i32(lds, qo + 32);
+      const int t6 = lds_u8(lds, to + 6), t8 = lds_u8(lds, to + 8);
+      const float n2 = pair(lds, q6, t6), nh2 = pairh(lds, q6, t6);
+      const float n4 = pair(lds, q8, t8), nh4 = pairh(lds, q8, t8);
       CTH_DDMS_4CELL(h0, s2, h2, s4, h4, fb)
-      const float p2 = pair(dmsT, q[1 + ib + 10], tj[ib + 10]), ph2 = pairh(dmsT, q[1 + ib + 10], tj[ib + 10]);
-      const float p4 = pair(dmsT, q[1 + ib + 12], tj[ib + 12]), ph4 = pairh(dmsT, q[1 + ib + 12], tj[ib + 12]);
+      const int q10 = lds_i32(lds, qo + 40), q12 = lds_i32(lds, qo + 48);
+      const int t10 = lds_u8(lds, to + 10), t12 = lds_u8(lds, to + 12);
+      const float p2 = pair(lds, q10, t10), ph2 = pairh(lds, q10, t10);
+      const float p4 = pair(lds, q12, t12), ph4 = pairh(lds, q12, t12);
       if (ib + 4 < iend) CTH_DDMS_4CELL(h4, n2, nh2, n4, nh4, fb + 4.0f * kPackF)
       fb += 8.0f * kPackF;
+      qo += 32;
+      to += 8;
+      asm("" : "+v"(qo), "+v"(to));
       h0 = nh4;
       s2 = p2;
       h2 = ph2;
@@ -276,24 +295,33 @@ __device__ __forceinline__ float ddms_cell(const float* __restrict__ dmsT, const
   }
   // lower part: diagonal i pairs query index i + j with template index j
   for (int i = len1 >> 2; 0 < i; i -= 2) {
-    const int32_t* qi = q + 1 + i;
     const int jend = min(len2, len1 - i);
-    const float s0 = pair(dmsT, qi[0], t[1]);
+    int qo = qr + 4 * (1 + i), to = tr + 1;
+    const float s0 = pair(lds, lds_i32(lds, qo), lds_u8(lds, to));
     float u = s0 + fabsf(s0);  // 2 max(s0, 0)
     float dd = 0.0f;
     // 2^23 + the packed coordinates of cell 1 (query i + 1, template 1)
     float fb = __uint_as_float(kMagic + kPack + (uint32_t)i);
-    float h0 = pairh(dmsT, qi[0], t[1]);
-    float s2 = pair(dmsT, qi[2], t[1 + 2]), h2 = pairh(dmsT, qi[2], t[1 + 2]);
-    float s4 = pair(dmsT, qi[4], t[1 + 4]), h4 = pairh(dmsT, qi[4], t[1 + 4]);
+    float h0 = pairh(lds, lds_i32(lds, qo), lds_u8(lds, to));
+    float s2 = pair(lds, lds_i32(lds, qo + 8), lds_u8(lds, to + 2)),
+          h2 = pairh(lds, lds_i32(lds, qo + 8), lds_u8(lds, to + 2));
+    float s4 = pair(lds, lds_i32(lds, qo + 16), lds_u8(lds, to + 4)),
+          h4 = pairh(lds, lds_i32(lds, qo + 16), lds_u8(lds, to + 4));
     for (int ib = 0; ib < jend; ib += 8) {
-      const float n2 = pair(dmsT, qi[ib + 6], t[1 + ib + 6]), nh2 = pairh(dmsT, qi[ib + 6], t[1 + ib + 6]);
-      const float n4 = pair(dmsT, qi[ib + 8], t[1 + ib + 8]), nh4 = pairh(dmsT, qi[ib + 8], t[1 + ib + 8]);
+      const int q6 = lds_i32(lds, qo + 24), q8 = lds_i32(lds, qo + 32);
+      const int t6 = lds_u8(lds, to + 6), t8 = lds_u8(lds, to + 8);
+      const float n2 = pair(lds, q6, t6), nh2 = pairh(lds, q6, t6);
+      const float n4 = pair(lds, q8, t8), nh4 = pairh(lds, q8, t8);
       CTH_DDMS_4CELL(h0, s2, h2, s4, h4, fb)
-      const float p2 = pair(dmsT, qi[ib + 10], t[1 + ib + 10]), ph2 = pairh(dmsT, qi[ib + 10], t[1 + ib + 10]);
-      const float p4 = pair(dmsT, qi[ib + 12], t[1 + ib + 12]), ph4 = pairh(dmsT, qi[ib + 12], t[1 + ib + 12]);
+      const int q10 = lds_i32(lds, qo + 40), q12 = lds_i32(lds, qo + 48);
+      const int t10 = lds_u8(lds, to + 10), t12 = lds_u8(lds, to + 12);
+      const float p2 = pair(lds, q10, t10), ph2 = pairh(lds, q10, t10);
+      const float p4 = pair(lds, q12, t12), ph4 = pairh(lds, q12, t12);
       if (ib + 4 < jend) CTH_DDMS_4CELL(h4, n2, nh2, n4, nh4, fb + 4.0f * kPackF)
       fb += 8.0f * kPackF;
+      qo += 32;
+      to += 8;
+      asm("" : "+v"(qo), "+v"(to));
       h0 = nh4;
       s2 = p2;
       h2 = ph2;
@@ -346,22 +374,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
   }
   for (int k = tid; k < 467; k += 256) L.ss[k] = tabs->ss[k];
   for (int k = tid; k < tabs->cvcard; k += 256) L.cvvals[k] = tabs->cvvals[k];
-  // stage 64 template lists (32 dwords each)
+  // stage 64 template lists (32 dwords each), values (<= 21) times 4: the
+  // pair table's byte column; the length byte stays
   for (int k = tid; k < kCols * 32; k += 256) {
     const int c = k >> 5, d = k & 31;
     const int cc = blockIdx.x * kCols + c;
     uint32_t v = 0x15151515u;  // padding value 21
     if (cc < ncols) v = reinterpret_cast<const uint32_t*>(db.dd + (size_t)(col0 + cc) * CTH_DDSTRIDE)[d];
     if (cc >= ncols && d == 0) v = 0x15151500u;  // empty list
+    v = d == 0 ? (v & 0xffu) | ((v & 0xffffff00u) << 2) : v << 2;
     *reinterpret_cast<uint32_t*>(&L.tl[c * kListPitch + d * 4]) = v;
+  }
+  // each wavefront's query list (table byte offsets; rows past the query
+  // stage its last row, whose results are not stored)
+  {
+    const int rr = min(row, lq - 1);
+    const int32_t* qsrc = qoff + (size_t)(q0 + rr) * CTH_DDSTRIDE;
+    for (int k = lane; k < kQlEntries; k += 64) L.ql[w][k] = qsrc[k];
   }
   __syncthreads();
   if (row >= lq || col >= ncols) return;
 
-  // DDMS first, while nothing else is live: the wavefront's query list (one
-  // row per wavefront) by scalar loads
-  const int urow = __builtin_amdgcn_readfirstlane(row);
-  const float d = ddms_cell(L.dms, qoff + (size_t)(q0 + urow) * CTH_DDSTRIDE, &L.tl[lane * kListPitch]);
+  // DDMS first, while nothing else is live
+  const float d = ddms_cell(reinterpret_cast<const char*>(&L), (int)offsetof(ScoreLds, ql) + w * kQlEntries * 4,
+                            lane * kListPitch);
 
   const int64_t qi = q0 + row;
   const int64_t tj = col0 + col;
