@@ -108,6 +108,9 @@ __device__ __forceinline__ void register_max(float& m0, float& m1, float& m2, ui
     m0 = s;
     nb0 = kEndBias - sb;
     e0 = se + kEndBias;
+    // new values rather than the candidate's registers, so that the join
+    // keeps m0 and its coordinates where they live without copies
+    asm("" : "+v"(m0), "+v"(nb0), "+v"(e0));
   } else if (m1 <= s) {
     // s becomes m1 (s does not intersect m0 when the register changes)
     m2 = i1 ? 0.0f : m1;
