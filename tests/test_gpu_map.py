@@ -25,6 +25,10 @@ def _check(ctx, oracle, q, db, qi, p0, p1, profiles, logepa):
         for f in FIELDS:
             g, w = res[j][f], want[f]
             assert np.asarray(g).tobytes() == np.asarray(w).tobytes(), (p, f, g, w)
+        # the reference's MAP threads also compute cells outside the matrix;
+        # each adds exp(-log Z) - minpp to its predecessor, so none of them
+        # can hold a maximum (neither model has them) while log Z > log(1/minpp)
+        assert res[j]["logz"] > np.log(1.0 / cother_amd.MINPP), (p, res[j]["logz"])
     return res
 
 
