@@ -21,6 +21,12 @@
 // the phase-1 DP keeps the reference's buffer of maxima (value and cell per
 // column) next to its five boundary rows: 28 bytes per column in LDS
 #define CTH_DP_LDS_ROW_MAX 5824
+// the per-query phase-1 DP keeps those rows in global memory from this
+// length on: a workgroup of one wavefront with 28 bytes of LDS per column
+// leaves most of a CU's SIMDs idle for long profiles
+#ifndef CTH_DP_GLOBAL_FROM
+#define CTH_DP_GLOBAL_FROM 1536
+#endif
 
 // transition order (cudp/CuBatchDP_com.h:87-96)
 enum { CTH_MM = 0, CTH_MI, CTH_MD, CTH_IM, CTH_II, CTH_DM, CTH_DD };

@@ -9,8 +9,10 @@
 // DPP wave_shr:1 move, so the wavefront advances one anti-diagonal per step
 // with no LDS round trip.  The bottom row of a stripe is kept in LDS for the
 // next stripe (in a global-memory row of the wavefront's own for profiles
-// longer than CTH_DP_LDS_ROW_MAX positions, whose five rows and buffer of
-// maxima exceed the LDS; the row of column x is read at step x + 31 and
+// longer than CTH_DP_GLOBAL_FROM positions in the per-query search, whose
+// five rows and buffer of maxima would leave a CU one or two single-wavefront
+// workgroups in LDS, and over CTH_DP_LDS_ROW_MAX, the LDS capacity, in the
+// batched search; the row of column x is read at step x + 31 and
 // overwritten at step x + 95 of the same wavefront, so one row serves both
 // stripes as in LDS).  Cell values
 // equal the reference's: rows start at column -1-(q mod 32) with zero
@@ -428,14 +430,14 @@ extern "C" int cth_selftest_dpp(int* host_out64) {
 // Bytes of global boundary rows a launch over np profiles of at most ltmax
 // positions needs (0 when the rows fit the LDS).
 extern "C" size_t cth_dp_row_bytes(int np, int ltmax) {
-  return ltmax > CTH_DP_LDS_ROW_MAX ? sizeof(float) * 7 * (size_t)ltmax * (size_t)(np > 0 ? np : 0) : 0;
+  return ltmax > CTH_DP_GLOBAL_FROM ? sizeof(float) * 7 * (size_t)ltmax * (size_t)(np > 0 ? np : 0) : 0;
 }
 
 // Phase 1 for np profiles: p0 + k, or plist[k] (device array of absolute
 // indices) when plist is not NULL; res[k] receives profile k's record.
 // btck NULL: no traceback bytes (maximum and its coordinates only);
 // finalize: run the finalisation (needs btck).  grow: cth_dp_row_bytes(np,
-// ltmax) bytes of device memory when ltmax exceeds CTH_DP_LDS_ROW_MAX (else
+// ltmax) bytes of device memory when ltmax exceeds CTH_DP_GLOBAL_FROM (else
 // unused).
 extern "C" int cth_launch_dp_list(const cth_dpset* q, int qp, const cth_dpset* db, int p0, const int* plist, int np,
                                   int ltmax, int64_t col0, int64_t ldsc, const float* scores, uint8_t* btck,
@@ -443,7 +445,7 @@ extern "C" int cth_launch_dp_list(const cth_dpset* q, int qp, const cth_dpset* d
   if (np <= 0) return 0;
   if (finalize && !btck) { cth_set_error("dp: finalisation needs the traceback"); return -1; }
   if (ltmax > CTH_MAXCOLUMNS) { cth_set_error("dp: profile length %d over %d", ltmax, CTH_MAXCOLUMNS); return -1; }
-  if (ltmax > CTH_DP_LDS_ROW_MAX) {
+  if (ltmax > CTH_DP_GLOBAL_FROM) {
     if (!grow) { cth_set_error("dp: profile length %d needs global boundary rows", ltmax); return -1; }
     hipLaunchKernelGGL(k_dp<true>, dim3(np), dim3(64), 0, st, *q, qp, *db, p0, plist, col0, ldsc, scores, btck, res,
                        grow, (int64_t)7 * ltmax);

@@ -348,7 +348,7 @@ int search_run(cth_ctx* c, int q0, int q1, const cth_search_opts* opt, cth_hit* 
       // in groups of consecutive profiles, each launch reserving LDS for its
       // own longest profile, so that groups of shorter profiles (databases
       // are sorted by length) run more wavefronts per CU.
-      // Profiles longer than CTH_DP_LDS_ROW_MAX keep their stripe boundary rows
+      // Profiles longer than CTH_DP_GLOBAL_FROM keep their stripe boundary rows
       // in the work area, in launches of as many profiles as their rows fit.
       constexpr int kDpGroup = 2048;
       for (int a = p0; a < p1; a += kDpGroup) {
@@ -357,7 +357,7 @@ int search_run(cth_ctx* c, int q0, int q1, const cth_search_opts* opt, cth_hit* 
         for (int p = a; p < b; p++) lg = std::max(lg, db.h_len[p]);
         int nb = b - a;
         float* grow = nullptr;
-        if (lg > CTH_DP_LDS_ROW_MAX) {
+        if (lg > CTH_DP_GLOBAL_FROM) {
           nb = bufs.row_batch(cth_dp_row_bytes(1, lg), 0, b - a);
           if (nb < 1) return -1;
           grow = (float*)bufs.work(cth_dp_row_bytes(nb, lg));
@@ -389,7 +389,7 @@ int search_run(cth_ctx* c, int q0, int q1, const cth_search_opts* opt, cth_hit* 
         res[k].passed = 0; res[k].score = 0.0f; res[k].psts = 0; res[k].beg = 0;
         if (scorethld < res[k].maxsc) {
           const int lt = db.h_len[p0 + k];
-          if (lt > CTH_DP_LDS_ROW_MAX) {
+          if (lt > CTH_DP_GLOBAL_FROM) {
             cand_long.push_back(k);
             ltl = std::max(ltl, lt);
           } else {
