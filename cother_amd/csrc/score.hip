@@ -72,7 +72,9 @@ constexpr uint32_t kNoBeginN = kEndBias - (kMagic + 0x0007f8ffu), kNoEnd = kMagi
 // the min of the ends (CuBatchScoreMatrix_ddms2s.cuh:141-153), i.e. (begin
 // <= end within each segment) ce >= sb and se >= cb
 __device__ __forceinline__ bool meets(uint32_t sb, uint32_t se, uint32_t nb, uint32_t ce) {
-  return ((ce - sb) & (se + nb) & kEndBias) != 0u;
+  // the three-input and as one instruction (left to itself the compiler
+  // shares (ce - sb) & kEndBias between tests at the cost of an extra and)
+  return __builtin_amdgcn_bitop3_b32(ce - sb, se + nb, kEndBias, 0x80) != 0u;
 }
 
 // RegisterMaxDDMScore (CuBatchScoreMatrix_ddms2s.cuh:168-261): keep the

@@ -72,3 +72,19 @@ def test_empty_database_and_profiles_without_distances(ctx, oracle):
     nodist = cother_amd.synth(5, 0, lmin=30, lmax=200, dist_mode=2, sort_desc=0, seed=99)
     print("edges: templates without distance lists", flush=True)
     assert _compare(ctx, oracle, q, nodist, 100.0) >= 0
+
+
+def test_distance_lists_over_99_values_refused(ctx):
+    """The reference's lists hold at most 99 values (MAX_N_IR_DISTANCE_VALUES,
+    libpro/srcpro/MOptions.h:15), and the score kernel's scan relies on it:
+    a longer list is refused on upload, for queries and for the database."""
+    ps = cother_amd.synth(3, 0, lmin=20, lmax=30, dist_mode=1, sort_desc=0, seed=5)
+    dd = ps.field("dd")
+    assert dd.shape[1] == 128
+    dd[7, 0] = 99
+    ctx.upload_db(ps)
+    ctx.upload_queries(ps)
+    dd[7, 0] = 100
+    for up in (ctx.upload_db, ctx.upload_queries):
+        with pytest.raises(RuntimeError, match="position 7: distance list of 100 values"):
+            up(ps)
