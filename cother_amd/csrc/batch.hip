@@ -52,8 +52,8 @@
 
 extern "C" int cth_launch_score_batch(const cth_dpset* q, const int32_t* qoff, const cth_dpset* db, int ncols,
                                       int64_t ldsc, const cth_tables* dtabs, const cth_bytile* ytab, int nytiles,
-                                      const cth_bquery* bq, float* scores, float* mods, hipStream_t st);
-extern "C" int cth_score_rows_per_tile(void);
+                                      const cth_bquery* bq, float* scores, float* mods, int ddms, hipStream_t st);
+extern "C" int cth_score_rows_per_tile(int ddms);
 extern "C" int cth_launch_dp_batch(const cth_dpset* q, const cth_dpset* db, const cth_bquery* bq,
                                    const cth_bpair* pairs, const int* list, const int* count, int n, int* next,
                                    int ltmax, int64_t ldsc, const float* scores, uint8_t* btck, cth_dpres* res,
@@ -191,6 +191,7 @@ struct batch_plan {
   std::vector<cth_bpair> pairs;
   int64_t cells = 0;
   int lqmax = 0;
+  int ddms = 1;                     // 0: no cell of the batch has a DDMS term
 };
 
 struct launch_shape {
@@ -210,7 +211,7 @@ int capture_or_launch(cth_ctx* c, const batch_plan& P, const batch_layout& L, co
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, st, L.ctr);
     CTH_HIP_CHECK(hipGetLastError());
     if (cth_launch_score_batch(&c->q.d, c->q.ddoff, &db.d, (int)db.d.npos, ldsc, c->dtab, L.ytab, ntiles, L.bq,
-                               L.scores, L.mods, st))
+                               L.scores, L.mods, P.ddms, st))
       return -1;
     if (timing) CTH_HIP_CHECK(hipEventRecord(c->ev[1], st));
     if (cth_launch_dp_batch(&c->q.d, &db.d, L.bq, L.pairs, nullptr, nullptr, npairs, L.ctr + kCtrNext1, ltmax, ldsc,
@@ -306,7 +307,6 @@ int search_batch_list(cth_ctx* c, const int* ql, int nq, const cth_search_opts* 
   const size_t budget = opt->mem_budget ? opt->mem_budget : ((size_t)64 << 30);
   const int64_t ldsc = (npos + 63) / 64 * 64;
   const size_t bytes_per_cell = opt->do_map ? 13 : 9;
-  const int rows = cth_score_rows_per_tile();
   const bool timing = opt->timing != 0;
   if (timing && !c->has_ev) {
     for (int k = 0; k < 4; k++) CTH_HIP_CHECK(hipEventCreate(&c->ev[k]));
@@ -327,6 +327,7 @@ int search_batch_list(cth_ctx* c, const int* ql, int nq, const cth_search_opts* 
   uint64_t alnused = 0;
   // batch-local index of each query of the current batch (for its hits)
   std::vector<int> local(c->q.d.nprof, -1);
+  const int rows4 = cth_score_rows_per_tile(1);
   for (int qa = 0; qa < nq;) {
     // the batch: consecutive entries of the list whose buffers fit the
     // budget together
@@ -338,7 +339,8 @@ int search_batch_list(cth_ctx* c, const int* ql, int nq, const cth_search_opts* 
     while (qb < nq) {
       const int lq = c->q.h_len[ql[qb]];
       const size_t qbytes = (size_t)lq * (size_t)ldsc * bytes_per_cell;
-      const int qt = (lq + rows - 1) / rows;
+      // row tiles of the 4-row score kernel (the plain kernel's taller tiles are fewer)
+      const int qt = (lq + rows4 - 1) / rows4;
       if (qb > qa && (bytes + qbytes > budget || tiles + qt > 65535)) break;
       bytes += qbytes;
       tiles += qt;
@@ -368,6 +370,13 @@ int search_batch_list(cth_ctx* c, const int* ql, int nq, const cth_search_opts* 
       qa++;
       continue;
     }
+    // no DDMS term in the batch when the database's or every query's
+    // distance lists are shorter than DSTSEGM: the plain score kernel and
+    // its taller row tiles
+    P.ddms = 0;
+    if (cth_devset_has_ddms(&db, 0, nprof))
+      for (int k = P.q0; k < P.q1 && !P.ddms; k++) P.ddms = cth_devset_has_ddms(&c->q, ql[k], ql[k] + 1);
+    const int rows = cth_score_rows_per_tile(P.ddms);
     for (int k = P.q0; k < P.q1; k++) {
       const int qi = ql[k];
       local[qi] = k - P.q0;
@@ -487,7 +496,7 @@ int search_batch_list(cth_ctx* c, const int* ql, int nq, const cth_search_opts* 
                         (uint64_t)P.cells, (uint64_t)P.lqmax | ((uint64_t)rowcap << 32),
                         (uint64_t)S.dp_grid | ((uint64_t)S.small_grid << 32),
                         (uint64_t)opt->do_map | ((uint64_t)opt->ssemodel << 8) | ((uint64_t)(alnbytes != 0) << 17) |
-                            ((uint64_t)mp << 32),
+                            ((uint64_t)P.ddms << 18) | ((uint64_t)mp << 32),
                         fnv(&db.d, sizeof db.d), fnv(&c->q.d, sizeof c->q.d),
                         (uint64_t)(uintptr_t)c->q.ddoff, (uint64_t)(uintptr_t)c->dtab};
     if (timing) {

@@ -207,7 +207,13 @@ extern "C" int cth_upload_queries_packed(cth_ctx* c, char* const* pmbeg, char* c
 
 extern "C" int cth_launch_score(const cth_dpset* q, const int32_t* qoff, int qp, const cth_dpset* db, int64_t col0, int ncols,
                                 int64_t ldsc, const cth_tables* dtabs, int lq, float* scores, float* mods, int contraction,
-                                hipStream_t st);
+                                int ddms, hipStream_t st);
+
+// whether a score stage of query qidx against profiles [p0, p1) can have a
+// nonzero DDMS term
+static int ddms_of(const cth_ctx* c, int qidx, int p0, int p1) {
+  return cth_devset_has_ddms(&c->q, qidx, qidx + 1) && cth_devset_has_ddms(&c->db, p0, p1);
+}
 
 // Score matrix of query qidx against template profiles [p0,p1), copied to
 // host arrays [Lq][ncols] (ncols = positions of those profiles).
@@ -227,7 +233,7 @@ extern "C" int cth_score_block_ex(cth_ctx* c, int qidx, int p0, int p1, int cont
   if (cth_ctx_reserve(c, (void**)&c->scores, &c->cap_scores, need)) return -1;
   if (cth_ctx_reserve(c, (void**)&c->mods, &c->cap_mods, need)) return -1;
   if (cth_launch_score(&c->q.d, c->q.ddoff, qidx, &c->db.d, col0, (int)ncols, ldsc, c->dtab, lq, c->scores, c->mods,
-                       contraction, c->st))
+                       contraction, ddms_of(c, qidx, p0, p1), c->st))
     return -1;
   CTH_HIP_CHECK(hipStreamSynchronize(c->st));
   CTH_HIP_CHECK(hipMemcpy2D(h_scores, sizeof(float) * ncols, c->scores, sizeof(float) * ldsc, sizeof(float) * ncols,
@@ -268,7 +274,8 @@ extern "C" int cth_phase1_block(cth_ctx* c, int qidx, int p0, int p1, float scor
   if (cth_ctx_reserve(c, (void**)&c->mods, &c->cap_mods, need)) return -1;
   if (cth_ctx_reserve(c, (void**)&c->btck, &c->cap_btck, (size_t)lq * ldsc)) return -1;
   if (cth_ctx_reserve(c, (void**)&c->dpout, &c->cap_dpout, sizeof(cth_dpres) * (size_t)(p1 - p0))) return -1;
-  if (cth_launch_score(&c->q.d, c->q.ddoff, qidx, &c->db.d, col0, (int)ncols, ldsc, c->dtab, lq, c->scores, c->mods, 0, c->st))
+  if (cth_launch_score(&c->q.d, c->q.ddoff, qidx, &c->db.d, col0, (int)ncols, ldsc, c->dtab, lq, c->scores, c->mods, 0,
+                       ddms_of(c, qidx, p0, p1), c->st))
     return -1;
   // boundary rows in global memory for profiles over the LDS row limit
   const size_t rows = cth_dp_row_bytes(p1 - p0, ltmax);
@@ -371,7 +378,8 @@ extern "C" int cth_map_block(cth_ctx* c, int qidx, int p0, int p1, const int32_t
   cth_mapjob* d_jobs = (cth_mapjob*)(d_zrow + (size_t)njobs * lq);
   cth_mapres* d_res = (cth_mapres*)(((uintptr_t)(d_jobs + njobs) + 15) & ~(uintptr_t)15);
   CTH_HIP_CHECK(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(cth_mapjob) * njobs, hipMemcpyHostToDevice, c->st));
-  if (cth_launch_score(&c->q.d, c->q.ddoff, qidx, &c->db.d, col0, (int)ncols, ldsc, c->dtab, lq, c->scores, c->mods, 0, c->st))
+  if (cth_launch_score(&c->q.d, c->q.ddoff, qidx, &c->db.d, col0, (int)ncols, ldsc, c->dtab, lq, c->scores, c->mods, 0,
+                       ddms_of(c, qidx, p0, p1), c->st))
     return -1;
   const size_t rows = cth_map_row_bytes(njobs, ltmax);
   if (rows && cth_ctx_reserve(c, (void**)&c->grow, &c->cap_grow, rows)) return -1;

@@ -9,6 +9,7 @@
 // (CacheIRDistanceValuesh2, cusco/CuBatchScoreMatrix_ddms2s_h2.cuh:55-108).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -40,6 +41,13 @@ int up_fields(void** dst, const float* src, int nf, int64_t n, int64_t ld) {
 
 }  // namespace
 
+int cth_devset_has_ddms(const cth_devset* s, int p0, int p1) {
+  if (!s->h_maxdd) return 1;
+  for (int p = p0; p < p1; p++)
+    if (s->h_maxdd[p] >= CTH_DSTSEGM) return 1;
+  return 0;
+}
+
 extern "C" void cth_devset_free(cth_devset* s) {
   if (!s) return;
   for (void* p : {(void*)s->d.len, (void*)s->d.dist, (void*)s->d.eno, (void*)s->d.bkg, (void*)s->d.aa, (void*)s->d.sss,
@@ -49,6 +57,7 @@ extern "C" void cth_devset_free(cth_devset* s) {
   if (s->ddoff) (void)hipFree(s->ddoff);
   free(s->h_len);
   free(s->h_dist);
+  free(s->h_maxdd);
   memset(s, 0, sizeof *s);
 }
 
@@ -63,6 +72,13 @@ extern "C" int cth_devset_upload(const cth_pset* h, cth_devset* s) {
                     (int)h->dd[(size_t)i * CTH_DDSTRIDE], CTH_MAXDST);
       return -1;
     }
+  s->h_maxdd = (int32_t*)calloc(h->nprof > 0 ? h->nprof : 1, sizeof(int32_t));
+  if (!s->h_maxdd) { cth_set_error("out of host memory"); return -1; }
+  for (int p = 0; p < h->nprof; p++) {
+    int m = 0;
+    for (int k = 0; k < h->len[p]; k++) m = std::max<int>(m, h->dd[(size_t)(h->dist[p] + k) * CTH_DDSTRIDE]);
+    s->h_maxdd[p] = m;
+  }
   const int64_t ntr = h->npos + h->nprof, ldtr = pad64(ntr > 0 ? ntr : 1);
   cth_dpset& d = s->d;
   d.nprof = h->nprof;

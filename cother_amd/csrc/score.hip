@@ -475,20 +475,144 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
   mods[row * ldsc + col] = ms;
 }
 
+// The score stage where no cell can have a DDMS term (every query or every
+// template list shorter than DSTSEGM: ddms_cell returns 0 and k_score adds
+// nothing), with the same per-cell arithmetic as k_score's log-odds, SS and
+// CVS terms.  Without the DDMS scan the cell is ~400 instructions and k_score's
+// 4-row tiles would re-read each template column's fields per 4 rows; here a
+// workgroup covers kPlainRows query rows (kPlainRows / 4 per wavefront) of 64
+// template columns, each lane holding its column's fields in registers and
+// the wavefront's query row read by scalar loads.
+constexpr int kPlainRows = 32;
+
+template <bool kBatch = false>
+__global__ __launch_bounds__(256) void k_score_plain(cth_dpset q, int qp, cth_dpset db, int64_t col0, int ncols,
+                                                     int64_t ldsc, const cth_tables* __restrict__ tabs,
+                                                     float* __restrict__ scores, float* __restrict__ mods,
+                                                     const cth_bytile* __restrict__ ytab = nullptr,
+                                                     const cth_bquery* __restrict__ bq = nullptr) {
+  __shared__ float Lss[467];
+  __shared__ float Lcv[CTH_CVMAXCARD];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  int row0 = blockIdx.y * kPlainRows;
+  if constexpr (kBatch) {
+    const cth_bytile t = ytab[blockIdx.y];
+    const cth_bquery Q = bq[t.b];
+    qp = Q.qp;
+    row0 = t.row0;
+    scores += Q.off;
+    mods += Q.off;
+  }
+  const int lq = q.len[qp];
+  const int64_t q0 = q.dist[qp];
+  const int col = blockIdx.x * kCols + lane;
+  for (int k = tid; k < 467; k += 256) Lss[k] = tabs->ss[k];
+  for (int k = tid; k < tabs->cvcard; k += 256) Lcv[k] = tabs->cvvals[k];
+  __syncthreads();
+  if (col >= ncols) return;
+
+  // this lane's template column: frequencies and the log-odds denominators
+  // (CuBatchScoreMatrix_Init.cu:129-144,173-185: 0.5 (bq(a) + bt(a)), per
+  // query and template profile), SS probabilities and their table columns,
+  // context vector, its prior and squared norm
+  const int64_t tj = col0 + col;
+  const int tp = db.pro[tj];
+  float tf[CTH_NAA], den[CTH_NAA];
+#pragma unroll
+  for (int a = 0; a < CTH_NAA; a++) {
+    tf[a] = db.trg[a * db.ld + tj];
+    den[a] = 0.5f * (q.bkg[qp * CTH_NAA + a] + db.bkg[tp * CTH_NAA + a]);
+  }
+  float ts[CTH_NSS];
+  int tc[CTH_NSS];
+#pragma unroll
+  for (int j = 0; j < CTH_NSS; j++) {
+    ts[j] = db.ss[j * db.ld + tj];
+    tc[j] = j * 10 + (int)(ts[j] * 9.999f);
+  }
+  float tcv[CTH_NCV];
+#pragma unroll
+  for (int i = 0; i < CTH_NCV; i++) tcv[i] = db.cv[i * db.ld + tj];
+  const float kap = tabs->cvkappa0inv;
+  const float tn = db.cvnorm2[tj] + 1.0f + kap;
+  const float tprior = db.cvprior[tj];
+
+  for (int k = 0; k < kPlainRows / 4; k++) {
+    const int row = row0 + w * (kPlainRows / 4) + k;
+    if (row >= lq) break;
+    const int64_t qi = q0 + row;
+    // log-odds
+    float s = 0.0f;
+#pragma unroll
+    for (int a = 0; a < CTH_NAA; a++) {
+      const float num = q.trg[a * q.ld + qi] * tf[a];
+      s = s + num / den[a];
+    }
+    s = cth_logf(s) - CTH_INITSHIFT;
+    // SS (CuBatchScoreMatrix_ssss.cuh:64-116)
+    float sss = 0.0f;
+#pragma unroll
+    for (int i = 0; i < CTH_NSS; i++) {
+      const float f1 = q.ss[i * q.ld + qi];
+      const int r = i * 10 + (int)(f1 * 9.999f);
+#pragma unroll
+      for (int j = 0; j < CTH_NSS; j++) sss = fmaf(f1 * ts[j], ss_lookup(Lss, r, tc[j]), sss);
+    }
+    {
+      const float f1 = q.ss[(CTH_NSS - 1) * q.ld + qi];
+      if (-99.f < sss) {
+        sss = fmaf((f1 + 0.1f) * (ts[CTH_NSS - 1] + 0.1f), 0.1111111f, sss);
+        sss -= CTH_SSSHIFT;
+        sss *= CTH_SSSWGT;
+      }
+    }
+    // CVS (CuBatchScoreMatrix_cvs2s.cuh:80-113)
+    float cvs = 0.0f;
+#pragma unroll
+    for (int i = 0; i < CTH_NCV; i++) cvs = fmaf(q.cv[i * q.ld + qi], tcv[i], cvs);
+    cvs += kap;
+    {
+      const float a = (q.cvnorm2[qi] + 1.0f + kap) * tn;
+      cvs = fmaf(-cvs, cvs, a);
+      cvs = fmaf(tabs->cvpowernu0, cth_logf(cvs), tabs->cvcterm) - q.cvprior[qi] - tprior;
+      const int card = tabs->cvcard;
+      float v;
+      if (cvs <= tabs->cvkeys[0]) v = Lcv[0];
+      else if (tabs->cvkeys[card - 1] <= cvs) v = Lcv[card - 1];
+      else v = Lcv[tabs->cvshift + (int)rintf(2.0f * cvs)];
+      cvs = v * CTH_CVSWGT;
+    }
+    const float sc = sss + cvs;
+    s = s + sc;
+    scores[row * ldsc + col] = s;
+    mods[row * ldsc + col] = fmaf(-CTH_CVSSHIFT, CTH_CVSWGT, sc);
+  }
+}
+
 }  // namespace
 
 // Launch the score stage for query profile qp against template positions
 // [col0, col0+ncols) of db; outputs are [Lq][ldsc].
 // contraction 0: the FP32 log-odds term of the reference; 1 / 2: the term
-// from the BF16 / FP16 matrix-core product (init_mfma.hip) first.
+// from the BF16 / FP16 matrix-core product (init_mfma.hip) first.  ddms 0:
+// no cell can have a DDMS term (the query's or every template's lists are
+// shorter than DSTSEGM), so the FP32 stage runs k_score_plain.
 extern "C" int cth_launch_init_mfma(const cth_dpset* q, int qp, const cth_dpset* db, int64_t col0, int ncols,
                                     int64_t ldsc, int lq, int prec, float* out, hipStream_t st);
 extern "C" int cth_launch_score(const cth_dpset* q, const int32_t* qoff, int qp, const cth_dpset* db, int64_t col0,
                                 int ncols, int64_t ldsc, const cth_tables* dtabs, int lq, float* scores, float* mods,
-                                int contraction, hipStream_t st) {
+                                int contraction, int ddms, hipStream_t st) {
   if (ncols <= 0 || lq <= 0) return 0;
   if (!qoff) { cth_set_error("score: query distance offsets not built"); return -1; }
   if (ldsc < ncols) { cth_set_error("score: ldsc %lld < ncols %d", (long long)ldsc, ncols); return -1; }
+  if (!ddms && !contraction) {
+    dim3 pgrid((ncols + kCols - 1) / kCols, (lq + kPlainRows - 1) / kPlainRows);
+    hipLaunchKernelGGL(k_score_plain<false>, pgrid, dim3(256), 0, st, *q, qp, *db, col0, ncols, ldsc, dtabs, scores,
+                       mods);
+    CTH_HIP_CHECK(hipGetLastError());
+    return 0;
+  }
   dim3 grid((ncols + kCols - 1) / kCols, (lq + kRows - 1) / kRows);
   if (contraction) {
     if (cth_launch_init_mfma(q, qp, db, col0, ncols, ldsc, lq, contraction, scores, st)) return -1;
@@ -502,22 +626,27 @@ extern "C" int cth_launch_score(const cth_dpset* q, const int32_t* qoff, int qp,
 }
 
 // The score stage of a batch of queries in one launch: nytiles row tiles
-// (ytab: query of the batch and first row, kRows rows each) against the
-// whole resident database (ncols positions from column 0); each query's
-// scores and modular scores at its offset (bq[].off, row stride ldsc).
+// (ytab: query of the batch and first row, cth_score_rows_per_tile(ddms)
+// rows each) against the whole resident database (ncols positions from
+// column 0); each query's scores and modular scores at its offset
+// (bq[].off, row stride ldsc); ddms 0 as in cth_launch_score.
 extern "C" int cth_launch_score_batch(const cth_dpset* q, const int32_t* qoff, const cth_dpset* db, int ncols,
                                       int64_t ldsc, const cth_tables* dtabs, const cth_bytile* ytab, int nytiles,
-                                      const cth_bquery* bq, float* scores, float* mods, hipStream_t st) {
+                                      const cth_bquery* bq, float* scores, float* mods, int ddms, hipStream_t st) {
   if (ncols <= 0 || nytiles <= 0) return 0;
   if (!qoff) { cth_set_error("score batch: query distance offsets not built"); return -1; }
   if (ldsc < ncols) { cth_set_error("score batch: ldsc %lld < ncols %d", (long long)ldsc, ncols); return -1; }
   if (nytiles > 65535) { cth_set_error("score batch: %d row tiles (at most 65535)", nytiles); return -1; }
   dim3 grid((ncols + kCols - 1) / kCols, nytiles);
-  hipLaunchKernelGGL((k_score<false, true>), grid, dim3(256), 0, st, *q, 0, *db, (int64_t)0, ncols, ldsc, dtabs,
-                     scores, mods, qoff, ytab, bq);
+  if (!ddms)
+    hipLaunchKernelGGL((k_score_plain<true>), grid, dim3(256), 0, st, *q, 0, *db, (int64_t)0, ncols, ldsc, dtabs,
+                       scores, mods, ytab, bq);
+  else
+    hipLaunchKernelGGL((k_score<false, true>), grid, dim3(256), 0, st, *q, 0, *db, (int64_t)0, ncols, ldsc, dtabs,
+                       scores, mods, qoff, ytab, bq);
   CTH_HIP_CHECK(hipGetLastError());
   return 0;
 }
 
-// rows of one tile of the score kernel
-extern "C" int cth_score_rows_per_tile(void) { return kRows; }
+// rows of one tile of the score kernel (ddms 0: of k_score_plain)
+extern "C" int cth_score_rows_per_tile(int ddms) { return ddms ? kRows : kPlainRows; }

@@ -33,7 +33,7 @@
 
 extern "C" int cth_launch_score(const cth_dpset* q, const int32_t* qoff, int qp, const cth_dpset* db, int64_t col0, int ncols,
                                 int64_t ldsc, const cth_tables* dtabs, int lq, float* scores, float* mods, int contraction,
-                                hipStream_t st);
+                                int ddms, hipStream_t st);
 extern "C" int cth_launch_dp_list(const cth_dpset* q, int qp, const cth_dpset* db, int p0, const int* plist, int np,
                                   int ltmax, int64_t col0, int64_t ldsc, const float* scores, uint8_t* btck,
                                   float scorethld, int finalize, cth_dpres* res, float* grow, hipStream_t st);
@@ -337,8 +337,11 @@ int search_run(cth_ctx* c, int q0, int q1, const cth_search_opts* opt, cth_hit* 
       cth_dpres* const dpout = bufs.dpout;
       const double t0 = now_ms();
       if (timing) CTH_HIP_CHECK(hipEventRecord(c->ev[0], c->st));
+      // no DDMS term anywhere in the chunk when the query's or every template's
+      // distance lists are shorter than DSTSEGM
+      const int ddms = cth_devset_has_ddms(&c->q, qi, qi + 1) && cth_devset_has_ddms(&db, p0, p1);
       if (cth_launch_score(&c->q.d, c->q.ddoff, qi, &db.d, col0, (int)ncols, ldsc, c->dtab, lq, scores, mods,
-                           opt->contraction, c->st))
+                           opt->contraction, ddms, c->st))
         return -1;
       if (timing) CTH_HIP_CHECK(hipEventRecord(c->ev[1], c->st));
       // phase 1 in two passes: the DP of every profile without traceback,

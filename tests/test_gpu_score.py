@@ -80,3 +80,28 @@ def test_score_matrix_adversarial_distance_lists(ctx, oracle):
         bad = np.argwhere(sc.view(np.int32) != osc.view(np.int32))
         assert bad.size == 0, f"query {qi}: {len(bad)} cells differ, first {bad[:5].tolist()}"
         assert (md.view(np.int32) == omd.view(np.int32)).all()
+
+
+@pytest.mark.parametrize("case", ["templates without lists", "query without lists", "lists shorter than DSTSEGM"])
+def test_score_matrix_without_ddms_bit_exact(ctx, oracle, case):
+    """Where no cell can have a DDMS term (every template's or the query's
+    distance lists shorter than DSTSEGM) the plain score kernel runs: its
+    tall tiles (32 query rows; queries of 1-100 positions cover partial and
+    several tiles) must give the oracle's matrices bit for bit, with
+    informative context vectors."""
+    from tests._cv import set_informative_cv
+    qmode, tmode = (0, 2) if case == "templates without lists" else (2, 1) if case == "query without lists" else (0, 1)
+    q = set_informative_cv(cother_amd.synth(4, 0, lmin=1, lmax=100, dist_mode=qmode, sort_desc=0, seed=121), 122)
+    db = set_informative_cv(cother_amd.synth(70, 0, lmin=5, lmax=300, dist_mode=tmode, sort_desc=1, seed=123), 124)
+    if case == "lists shorter than DSTSEGM":
+        dd = db.field("dd")
+        dd[:, 0] = np.minimum(dd[:, 0], 5)
+    ctx.upload_queries(q)
+    ctx.upload_db(db)
+    for qi in range(q.nprof):
+        sc, md = ctx.score_block(qi, 0, db.nprof)
+        osc, omd = oracle.score_matrix(q, qi, db, 0, db.nprof)
+        assert sc.shape == osc.shape
+        bad = np.argwhere(sc.view(np.int32) != osc.view(np.int32))
+        assert bad.size == 0, f"{case}, query {qi}: {len(bad)} cells differ, first {bad[:5].tolist()}"
+        assert (md.view(np.int32) == omd.view(np.int32)).all()
