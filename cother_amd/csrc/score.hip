@@ -491,7 +491,9 @@ __global__ __launch_bounds__(256) void k_score_plain(cth_dpset q, int qp, cth_dp
                                                      float* __restrict__ scores, float* __restrict__ mods,
                                                      const cth_bytile* __restrict__ ytab = nullptr,
                                                      const cth_bquery* __restrict__ bq = nullptr) {
-  __shared__ float Lss[467];
+  // the SS table square (30 x 30, symmetric: entry r*30+c is ss_lookup(r, c)),
+  // so that a lookup is one index add
+  __shared__ float Lsq[30 * 30];
   __shared__ float Lcv[CTH_CVMAXCARD];
   const int tid = threadIdx.x;
   const int lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -507,7 +509,7 @@ __global__ __launch_bounds__(256) void k_score_plain(cth_dpset q, int qp, cth_dp
   const int lq = q.len[qp];
   const int64_t q0 = q.dist[qp];
   const int col = blockIdx.x * kCols + lane;
-  for (int k = tid; k < 467; k += 256) Lss[k] = tabs->ss[k];
+  for (int k = tid; k < 30 * 30; k += 256) Lsq[k] = ss_lookup(tabs->ss, k / 30, k % 30);
   for (int k = tid; k < tabs->cvcard; k += 256) Lcv[k] = tabs->cvvals[k];
   __syncthreads();
   if (col >= ncols) return;
@@ -555,9 +557,9 @@ __global__ __launch_bounds__(256) void k_score_plain(cth_dpset q, int qp, cth_dp
 #pragma unroll
     for (int i = 0; i < CTH_NSS; i++) {
       const float f1 = q.ss[i * q.ld + qi];
-      const int r = i * 10 + (int)(f1 * 9.999f);
+      const int r30 = (i * 10 + (int)(f1 * 9.999f)) * 30;
 #pragma unroll
-      for (int j = 0; j < CTH_NSS; j++) sss = fmaf(f1 * ts[j], ss_lookup(Lss, r, tc[j]), sss);
+      for (int j = 0; j < CTH_NSS; j++) sss = fmaf(f1 * ts[j], Lsq[r30 + tc[j]], sss);
     }
     {
       const float f1 = q.ss[(CTH_NSS - 1) * q.ld + qi];
