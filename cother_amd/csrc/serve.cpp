@@ -30,6 +30,7 @@
 // Results are those of cth_search for the same query: requests do not
 // share buffers, and every kernel's output depends only on its own inputs.
 #include <stdint.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -149,20 +150,34 @@ struct cth_server {
     std::vector<int> nh(n, 0);
     cth_search_stats st{};
     const int rc = cth_search_batch_list(c, ql.data(), n, &so, hits.data(), (int)hits.size(), nh.data(), &st);
-    const std::string err = rc ? cth_last_error() : "";
+    if (rc) {
+      // a failure of the batch (one request's rows not fitting, say) is
+      // reported per request: each runs alone, as it would without batching
+      for (Request* r : g) run_one(c, r);
+      return;
+    }
     size_t at = 0;
     for (int k = 0; k < n; k++) {
       Request* r = g[k];
-      r->rc = rc;
-      r->err = err;
+      r->rc = 0;
+      r->err.clear();
       r->stats = st;
-      if (rc) continue;
       r->hits.assign(hits.begin() + at, hits.begin() + at + nh[k]);
       at += nh[k];
       if (opt.alncap) {
-        // this request's rows, with offsets into its own buffer
+        // this request's rows, with offsets into its own buffer; rows over
+        // the per-request capacity fail the request as cth_search fails it
         uint64_t used = 0;
         for (cth_hit& h : r->hits) used += 5 * (uint64_t)h.alnlen;
+        if (used > opt.alncap) {
+          char msg[96];
+          snprintf(msg, sizeof msg, "search: alignment buffer full (%llu bytes)", (unsigned long long)opt.alncap);
+          r->rc = -1;
+          r->err = msg;
+          r->hits.clear();
+          r->aln.clear();
+          continue;
+        }
         r->aln.resize(std::max<uint64_t>(used, 1));
         used = 0;
         for (cth_hit& h : r->hits) {

@@ -110,3 +110,39 @@ def test_uploads_refused_while_serving(ctx, sets):
     finally:
         srv.close()
     ctx.upload_db(db)  # allowed again once the server is gone
+
+
+@pytest.mark.parametrize("which", ["largest", "smallest"])
+def test_batched_tick_failure_stays_per_request(ctx, sets, which):
+    """requests whose alignment rows overflow the per-request capacity fail as
+    they do alone ("alignment buffer full"), and the requests batched with
+    them get their results as alone: with the capacity just under the largest
+    request's rows the batch itself fits; with it at the smallest request's
+    rows the batch overflows and its requests run again one by one"""
+    q, db = sets
+    ctx.upload_queries(q)
+    ctx.upload_db(db)
+    want = [_alone(ctx, qi) for qi in range(q.nprof)]
+    need = [int(5 * want[qi][0]["alnlen"].astype(np.int64).sum()) for qi in range(q.nprof)]
+    cap = max(need) - 1 if which == "largest" else min(n for n in need if n > 0)
+    over = [n > cap for n in need]
+    assert cap > 0 and any(over) and not all(over)
+    if which == "smallest":
+        assert sum(need) > cap * q.nprof  # the batch's shared buffer overflows too
+    print(f"row bytes per query {need}, capacity {cap}", flush=True)
+    srv = cother_amd.Server(ctx, nslots=1, mem_budget=1 << 34, chunk_cols=0, alncap=cap, batch=q.nprof)
+    try:
+        tickets = [(qi, srv.submit(qi)) for qi in range(q.nprof)]
+        for qi, t in tickets:
+            if over[qi]:
+                with pytest.raises(RuntimeError, match="alignment buffer full"):
+                    srv.wait(t)
+                continue
+            hits, alns, info = srv.wait(t)
+            wh, wa = want[qi]
+            assert len(hits) == len(wh), f"query {qi}"
+            for f in ("profile", "score", "logE", "beg", "end", "mapbeg", "mapend", "alnlen", "idnts", "psts"):
+                assert (hits[f].view(np.uint32) == wh[f].view(np.uint32)).all(), f"query {qi} field {f}"
+            assert alns == wa, f"query {qi}"
+    finally:
+        srv.close()
