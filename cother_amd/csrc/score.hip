@@ -478,23 +478,50 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
 // The score stage where no cell can have a DDMS term (every query or every
 // template list shorter than DSTSEGM: ddms_cell returns 0 and k_score adds
 // nothing), with the same per-cell arithmetic as k_score's log-odds, SS and
-// CVS terms.  Without the DDMS scan the cell is ~400 instructions and k_score's
-// 4-row tiles would re-read each template column's fields per 4 rows; here a
-// workgroup covers kPlainRows query rows (kPlainRows / 4 per wavefront) of 64
-// template columns, each lane holding its column's fields in registers and
-// the wavefront's query row read by scalar loads.
+// CVS terms.  Without the DDMS scan the cell is a few hundred instructions
+// and k_score's 4-row tiles would re-read each template column's fields per
+// 4 rows; here a workgroup covers kPlainRows query rows (kPlainRows / 4 per
+// wavefront) of 64 template columns, each lane holding its column's fields
+// in registers.  The workgroup's query rows are staged in LDS, one 48-float
+// record per row, and a wavefront reads its row by broadcast: its values
+// are vector operands (scalar-register operands would keep the FP32
+// arithmetic from dual-issuing) and a row costs no scalar loads or address
+// arithmetic.
 constexpr int kPlainRows = 32;
+constexpr int kQrec = 48;  // floats per staged query row
+constexpr int kQvals = 44; // 20 frequencies, 3 SS, 19 context-vector entries, norm^2, prior
+
+// a factor of a log-odds numerator the quotient correction below admits:
+// +0 or [2^-32, 2], compared on the bit pattern
+__device__ __forceinline__ bool in_quot_range(float v) {
+  const uint32_t b = __builtin_bit_cast(uint32_t, v);
+  return b == 0u || b - 0x2F800000u <= 0x10800000u;
+}
+
+// value v (0..kQvals-1) of the staged record of query position qi
+__device__ __forceinline__ float query_value(const cth_dpset& q, int v, int64_t qi) {
+  if (v < CTH_NAA) return q.trg[v * q.ld + qi];
+  if (v < CTH_NAA + CTH_NSS) return q.ss[(v - CTH_NAA) * q.ld + qi];
+  if (v < CTH_NAA + CTH_NSS + CTH_NCV) return q.cv[(v - CTH_NAA - CTH_NSS) * q.ld + qi];
+  return v == kQvals - 2 ? q.cvnorm2[qi] : q.cvprior[qi];
+}
 
 template <bool kBatch = false>
-__global__ __launch_bounds__(256) void k_score_plain(cth_dpset q, int qp, cth_dpset db, int64_t col0, int ncols,
-                                                     int64_t ldsc, const cth_tables* __restrict__ tabs,
-                                                     float* __restrict__ scores, float* __restrict__ mods,
-                                                     const cth_bytile* __restrict__ ytab = nullptr,
-                                                     const cth_bquery* __restrict__ bq = nullptr) {
+// Five wavefronts per SIMD (at most 96 VGPRs): the template frequencies are
+// read from LDS per row, so that the column's denominators and their
+// reciprocals fit in registers beside the row's values.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_score_plain(
+    cth_dpset q, int qp, cth_dpset db, int64_t col0, int ncols, int64_t ldsc, const cth_tables* __restrict__ tabs,
+    float* __restrict__ scores, float* __restrict__ mods, const cth_bytile* __restrict__ ytab = nullptr,
+    const cth_bquery* __restrict__ bq = nullptr) {
   // the SS table square (30 x 30, symmetric: entry r*30+c is ss_lookup(r, c)),
   // so that a lookup is one index add
   __shared__ float Lsq[30 * 30];
   __shared__ float Lcv[CTH_CVMAXCARD];
+  __shared__ float4 Ltcv[5][kCols];
+  __shared__ float4 Ltf[5][kCols];
+  __shared__ __align__(16) float Lq[kPlainRows][kQrec];
+  __shared__ int Lbad[kPlainRows];  // a row has a frequency outside in_quot_range
   const int tid = threadIdx.x;
   const int lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
   int row0 = blockIdx.y * kPlainRows;
@@ -509,22 +536,46 @@ __global__ __launch_bounds__(256) void k_score_plain(cth_dpset q, int qp, cth_dp
   const int lq = q.len[qp];
   const int64_t q0 = q.dist[qp];
   const int col = blockIdx.x * kCols + lane;
+  const bool colin = col < ncols;
   for (int k = tid; k < 30 * 30; k += 256) Lsq[k] = ss_lookup(tabs->ss, k / 30, k % 30);
   for (int k = tid; k < tabs->cvcard; k += 256) Lcv[k] = tabs->cvvals[k];
+  if (tid < kPlainRows) Lbad[tid] = 0;
   __syncthreads();
-  if (col >= ncols) return;
 
-  // this lane's template column: frequencies and the log-odds denominators
+  // the query rows, read along the rows (rows past the query's end as zeros)
+  for (int k = tid; k < kPlainRows * kQvals; k += 256) {
+    const int r = k % kPlainRows, v = k / kPlainRows;
+    const float x = row0 + r < lq ? query_value(q, v, q0 + row0 + r) : 0.0f;
+    Lq[r][v] = x;
+    if (v < CTH_NAA && !in_quot_range(x)) Lbad[r] = 1;
+  }
+
+  // this lane's template column (lanes past the last column read column 0
+  // and leave after the staging): frequencies and the log-odds denominators
   // (CuBatchScoreMatrix_Init.cu:129-144,173-185: 0.5 (bq(a) + bt(a)), per
   // query and template profile), SS probabilities and their table columns,
-  // context vector, its prior and squared norm
-  const int64_t tj = col0 + col;
+  // context vector, its prior and squared norm.  The frequencies and the
+  // context vectors go to LDS, staged by wavefront 0 for the workgroup's 64
+  // columns, four values a lane per 16-byte slot.
+  const int64_t tj = col0 + (colin ? col : 0);
   const int tp = db.pro[tj];
   float tf[CTH_NAA], den[CTH_NAA];
 #pragma unroll
   for (int a = 0; a < CTH_NAA; a++) {
     tf[a] = db.trg[a * db.ld + tj];
     den[a] = 0.5f * (q.bkg[qp * CTH_NAA + a] + db.bkg[tp * CTH_NAA + a]);
+  }
+  if (w == 0) {
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+      float4 v;
+      v.x = db.cv[(4 * i) * db.ld + tj];
+      v.y = db.cv[(4 * i + 1) * db.ld + tj];
+      v.z = db.cv[(4 * i + 2) * db.ld + tj];
+      v.w = 4 * i + 3 < CTH_NCV ? db.cv[(4 * i + 3) * db.ld + tj] : 0.0f;
+      Ltcv[i][lane] = v;
+      Ltf[i][lane] = make_float4(tf[4 * i], tf[4 * i + 1], tf[4 * i + 2], tf[4 * i + 3]);
+    }
   }
   float ts[CTH_NSS];
   int tc[CTH_NSS];
@@ -533,36 +584,66 @@ __global__ __launch_bounds__(256) void k_score_plain(cth_dpset q, int qp, cth_dp
     ts[j] = db.ss[j * db.ld + tj];
     tc[j] = j * 10 + (int)(ts[j] * 9.999f);
   }
-  float tcv[CTH_NCV];
-#pragma unroll
-  for (int i = 0; i < CTH_NCV; i++) tcv[i] = db.cv[i * db.ld + tj];
   const float kap = tabs->cvkappa0inv;
   const float tn = db.cvnorm2[tj] + 1.0f + kap;
   const float tprior = db.cvprior[tj];
+  // The log-odds quotients num / den[a] by Markstein's correction: with
+  // y = RN(1 / den) (an IEEE division, once per lane), q0 = RN(num y),
+  // r = fma(-den, q0, num) (exact) and q = RN(q0 + r y) equal RN(num / den)
+  // when no step underflows or overflows (P. Markstein, IBM J. Res. Dev.
+  // 34:111-119, 1990): here when den lies in [2^-30, 2^30] and both factors
+  // of num are 0 or in [2^-32, 2].  A wavefront whose columns or query row
+  // have an operand outside those ranges divides.
+  float y[CTH_NAA];
+  bool lane_ok = true;
+#pragma unroll
+  for (int a = 0; a < CTH_NAA; a++) {
+    y[a] = 1.0f / den[a];
+    lane_ok &= in_quot_range(tf[a]) && __builtin_bit_cast(uint32_t, den[a]) - 0x30800000u <= 0x1E000000u;
+  }
+  __syncthreads();
+  if (!colin) return;
+  const bool cols_ok = __builtin_amdgcn_ballot_w64(!lane_ok) == 0;
 
   for (int k = 0; k < kPlainRows / 4; k++) {
-    const int row = row0 + w * (kPlainRows / 4) + k;
+    const int rl = w * (kPlainRows / 4) + k;
+    const int row = row0 + rl;
     if (row >= lq) break;
-    const int64_t qi = q0 + row;
+    const float* const R = Lq[rl];
     // log-odds
-    float s = 0.0f;
+    float tfr[CTH_NAA];
 #pragma unroll
-    for (int a = 0; a < CTH_NAA; a++) {
-      const float num = q.trg[a * q.ld + qi] * tf[a];
-      s = s + num / den[a];
+    for (int i4 = 0; i4 < 5; i4++) {
+      const float4 v = Ltf[i4][lane];
+      tfr[4 * i4] = v.x;
+      tfr[4 * i4 + 1] = v.y;
+      tfr[4 * i4 + 2] = v.z;
+      tfr[4 * i4 + 3] = v.w;
+    }
+    float s = 0.0f;
+    if (cols_ok && __builtin_amdgcn_readfirstlane(Lbad[rl]) == 0) {
+#pragma unroll
+      for (int a = 0; a < CTH_NAA; a++) {
+        const float num = R[a] * tfr[a];
+        const float q0 = num * y[a];
+        s = s + fmaf(fmaf(-den[a], q0, num), y[a], q0);
+      }
+    } else {
+#pragma unroll
+      for (int a = 0; a < CTH_NAA; a++) s = s + R[a] * tfr[a] / den[a];
     }
     s = cth_logf(s) - CTH_INITSHIFT;
     // SS (CuBatchScoreMatrix_ssss.cuh:64-116)
     float sss = 0.0f;
 #pragma unroll
     for (int i = 0; i < CTH_NSS; i++) {
-      const float f1 = q.ss[i * q.ld + qi];
+      const float f1 = R[CTH_NAA + i];
       const int r30 = (i * 10 + (int)(f1 * 9.999f)) * 30;
 #pragma unroll
       for (int j = 0; j < CTH_NSS; j++) sss = fmaf(f1 * ts[j], Lsq[r30 + tc[j]], sss);
     }
     {
-      const float f1 = q.ss[(CTH_NSS - 1) * q.ld + qi];
+      const float f1 = R[CTH_NAA + CTH_NSS - 1];
       if (-99.f < sss) {
         sss = fmaf((f1 + 0.1f) * (ts[CTH_NSS - 1] + 0.1f), 0.1111111f, sss);
         sss -= CTH_SSSHIFT;
@@ -572,12 +653,19 @@ __global__ __launch_bounds__(256) void k_score_plain(cth_dpset q, int qp, cth_dp
     // CVS (CuBatchScoreMatrix_cvs2s.cuh:80-113)
     float cvs = 0.0f;
 #pragma unroll
-    for (int i = 0; i < CTH_NCV; i++) cvs = fmaf(q.cv[i * q.ld + qi], tcv[i], cvs);
+    for (int i4 = 0; i4 < 5; i4++) {
+      const float4 v = Ltcv[i4][lane];
+      const float* const c = R + CTH_NAA + CTH_NSS + 4 * i4;
+      cvs = fmaf(c[0], v.x, cvs);
+      cvs = fmaf(c[1], v.y, cvs);
+      cvs = fmaf(c[2], v.z, cvs);
+      if (4 * i4 + 3 < CTH_NCV) cvs = fmaf(c[3], v.w, cvs);
+    }
     cvs += kap;
     {
-      const float a = (q.cvnorm2[qi] + 1.0f + kap) * tn;
+      const float a = (R[kQvals - 2] + 1.0f + kap) * tn;
       cvs = fmaf(-cvs, cvs, a);
-      cvs = fmaf(tabs->cvpowernu0, cth_logf(cvs), tabs->cvcterm) - q.cvprior[qi] - tprior;
+      cvs = fmaf(tabs->cvpowernu0, cth_logf(cvs), tabs->cvcterm) - R[kQvals - 1] - tprior;
       const int card = tabs->cvcard;
       float v;
       if (cvs <= tabs->cvkeys[0]) v = Lcv[0];

@@ -82,20 +82,34 @@ def test_score_matrix_adversarial_distance_lists(ctx, oracle):
         assert (md.view(np.int32) == omd.view(np.int32)).all()
 
 
-@pytest.mark.parametrize("case", ["templates without lists", "query without lists", "lists shorter than DSTSEGM"])
+@pytest.mark.parametrize("case", ["templates without lists", "query without lists", "lists shorter than DSTSEGM",
+                                  "operands outside the quotient range"])
 def test_score_matrix_without_ddms_bit_exact(ctx, oracle, case):
     """Where no cell can have a DDMS term (every template's or the query's
     distance lists shorter than DSTSEGM) the plain score kernel runs: its
     tall tiles (32 query rows; queries of 1-100 positions cover partial and
     several tiles) must give the oracle's matrices bit for bit, with
-    informative context vectors."""
+    informative context vectors.  The kernel's log-odds quotients take a
+    division-free form for frequencies of 0 or in [2^-32, 2] and denominators
+    in [2^-30, 2^30]; the last case plants zeros (admitted), frequencies of
+    1e-12 and 3e-11 in some query rows and template columns and a background
+    of 1e-12 in one query and one template profile (not admitted: those rows
+    and wavefronts divide), so both forms run beside each other."""
     from tests._cv import set_informative_cv
-    qmode, tmode = (0, 2) if case == "templates without lists" else (2, 1) if case == "query without lists" else (0, 1)
+    qmode, tmode = (2, 1) if case == "query without lists" else (0, 1) if case == "lists shorter than DSTSEGM" else (0, 2)
     q = set_informative_cv(cother_amd.synth(4, 0, lmin=1, lmax=100, dist_mode=qmode, sort_desc=0, seed=121), 122)
     db = set_informative_cv(cother_amd.synth(70, 0, lmin=5, lmax=300, dist_mode=tmode, sort_desc=1, seed=123), 124)
     if case == "lists shorter than DSTSEGM":
         dd = db.field("dd")
         dd[:, 0] = np.minimum(dd[:, 0], 5)
+    if case == "operands outside the quotient range":
+        qt, tt = q.field("trg"), db.field("trg")
+        qt[3, ::7] = 0.0
+        qt[5, ::11] = 1e-12
+        tt[2, ::5] = 0.0
+        tt[4, 3::97] = 3e-11
+        q.field("bkg")[1, 6] = 1e-12
+        db.field("bkg")[3, 6] = 1e-12
     ctx.upload_queries(q)
     ctx.upload_db(db)
     for qi in range(q.nprof):
