@@ -36,6 +36,20 @@ __device__ __forceinline__ int flow_write_column(int x0, int tx, int lt) {
   return x0 + tx + ndx;
 }
 
+// Between the two half-wavefront phases of a block end (the lower half's
+// buffer writes, then the upper half's compare-and-write): the phases run
+// under exclusive lane conditions (hi == 0, hi != 0), so reasoning per
+// thread, a compiler may sink the first phase's stores into the second's
+// and issue both halves' stores as one instruction after the upper half's
+// reads (hipcc did so with the DP's step loop unrolled).  An inline asm that
+// may read and write any memory can neither be crossed by a store nor, being
+// convergent, be duplicated into the branches.  The fence keeps the
+// memory-model ordering; a wavefront's LDS accesses execute in issue order.
+__device__ __forceinline__ void flow_phase() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  asm volatile("" ::: "memory");
+}
+
 // the finalisation's reduction of the last 64 buffer columns (bmv values,
 // bmc cells): column lt-1-l against lt-33-l, then halves of 32 .. 1, a tie
 // keeping the lower index (the later column); the result reaches every lane

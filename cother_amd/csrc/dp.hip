@@ -252,12 +252,12 @@ __device__ __forceinline__ void dp_pair(const cth_dpset& q, int qp, const cth_dp
           bmv[w] = mv;
           bmc[w] = mc;
         }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        flow_phase();
         if (wr && hi != 0 && bmv[w] < mv) {
           bmv[w] = mv;
           bmc[w] = mc;
         }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        flow_phase();
       }
     }
     __syncthreads();

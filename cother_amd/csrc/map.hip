@@ -391,12 +391,12 @@ __device__ __forceinline__ void map_align_one(const cth_dpset& q, int qp, const 
           bmv[w] = mv;
           bmc[w] = mc;
         }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        flow_phase();
         if (wr && hi != 0 && bmv[w] < mv) {
           bmv[w] = mv;
           bmc[w] = mc;
         }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        flow_phase();
       }
     }
     __syncthreads();
