@@ -480,14 +480,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
 // nothing), with the same per-cell arithmetic as k_score's log-odds, SS and
 // CVS terms.  Without the DDMS scan the cell is a few hundred instructions
 // and k_score's 4-row tiles would re-read each template column's fields per
-// 4 rows; here a workgroup covers kPlainRows query rows (kPlainRows / 4 per
-// wavefront) of 64 template columns, each lane holding its column's fields
+// 4 rows; here a workgroup covers kPlainRows query rows (every fourth row
+// per wavefront) of 64 template columns, each lane holding its column's fields
 // in registers.  The workgroup's query rows are staged in LDS, one 48-float
 // record per row, and a wavefront reads its row by broadcast: its values
 // are vector operands (scalar-register operands would keep the FP32
 // arithmetic from dual-issuing) and a row costs no scalar loads or address
 // arithmetic.
-constexpr int kPlainRows = 32;
+constexpr int kPlainRows = 64;
 constexpr int kQrec = 48;  // floats per staged query row
 constexpr int kQvals = 44; // 20 frequencies, 3 SS, 19 context-vector entries, norm^2, prior
 
@@ -606,7 +606,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
   const bool cols_ok = __builtin_amdgcn_ballot_w64(!lane_ok) == 0;
 
   for (int k = 0; k < kPlainRows / 4; k++) {
-    const int rl = w * (kPlainRows / 4) + k;
+    const int rl = k * 4 + w;  // rows interleaved over the wavefronts: partial tiles stay balanced
     const int row = row0 + rl;
     if (row >= lq) break;
     const float* const R = Lq[rl];

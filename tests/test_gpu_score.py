@@ -87,7 +87,7 @@ def test_score_matrix_adversarial_distance_lists(ctx, oracle):
 def test_score_matrix_without_ddms_bit_exact(ctx, oracle, case):
     """Where no cell can have a DDMS term (every template's or the query's
     distance lists shorter than DSTSEGM) the plain score kernel runs: its
-    tall tiles (32 query rows; queries of 1-100 positions cover partial and
+    tall tiles (64 query rows; queries of 1-100 positions cover partial and
     several tiles) must give the oracle's matrices bit for bit, with
     informative context vectors.  The kernel's log-odds quotients take a
     division-free form for frequencies of 0 or in [2^-32, 2] and denominators
