@@ -461,7 +461,10 @@ int search_batch_list(cth_ctx* c, const int* ql, int nq, const cth_search_opts* 
         }
     }
     if (!tuned) {
-      const launch_shape cand[] = {{ncu * 8, ncu * 4}, {ncu * 20, ncu * 8}, {ncu * 40, ncu * 16}};
+      // wavefronts per compute unit for the phase-1 DP: measured on 128,000
+      // pairs of <= 150 x <= 400 positions (LDS for 14 per unit), 4: 47.5 ms,
+      // 6: 32.5, 8: 25.9, 10: 23.8, 12: 28.2, 14: 29.4
+      const launch_shape cand[] = {{ncu * 8, ncu * 4}, {ncu * 10, ncu * 4}, {ncu * 20, ncu * 8}};
       hipEvent_t e0, e1;
       CTH_HIP_CHECK(hipEventCreate(&e0));
       CTH_HIP_CHECK(hipEventCreate(&e1));
