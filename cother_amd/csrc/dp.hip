@@ -157,11 +157,40 @@ __device__ __forceinline__ void dp_pair(const cth_dpset& q, int qp, const cth_dp
     // test is applied where it is used)
     const float* __restrict__ srow = scores + (int64_t)y * ldsc + colp;
     auto load_sc = [&](int c) { return (rowok && 0 <= c && c < lt) ? srow[c] : 0.0f; };
-    load_col(-32, ntn, bn);
-    float sc1 = load_sc(-lane - 32), sc2 = load_sc(-lane - 31);
+    // The scores come in groups of four columns per lane, group G of lane l
+    // being columns 4G - l .. 4G - l + 3: at step t every lane is at element
+    // (t - 32) mod 4 of group (t - 32) / 4, so the whole wavefront refills at
+    // once, one 16-byte load (dword-aligned) per lane every four steps instead
+    // of four 4-byte loads; each lane reads its own row, so a load touches a
+    // cache line per lane, and fewer, wider loads take the texture path and
+    // L1 far fewer requests per cell.  Groups reaching outside the profile
+    // load their columns one by one (0 outside).
+    typedef float float4u __attribute__((ext_vector_type(4), aligned(4)));
+    auto load_grp = [&](int G, float* v) {
+      const int cs = 4 * G - lane;
+      if (rowok && 0 <= cs && cs + 3 < lt) {
+        const float4u x = *reinterpret_cast<const float4u*>(srow + cs);
+        v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
+      } else {
+        v[0] = load_sc(cs); v[1] = load_sc(cs + 1); v[2] = load_sc(cs + 2); v[3] = load_sc(cs + 3);
+      }
+    };
+    // Steps 0..30 compute no cell (lane l's first cell, at column
+    // -1-(l mod 32), comes at step 31 or later), write nothing and leave the
+    // state as it starts (dt log 0, zero states; no block starts or ends
+    // there), so the sweep starts at step 31 with its loads issued for it.
+    // After lane 63's last column (step lt + 94) only the final block end
+    // (step 32 ceil(lt/32) + 94, the next step = 30 mod 32, with no block
+    // start before it) remains, so the sweep jumps there.
+    constexpr int t0 = 31;
+    load_col(t0 - 32, ntn, bn);
+    float grp[4], grpn[4];  // this step's group and the next
+    load_grp((t0 - 32) >> 2, grp);
+    load_grp(((t0 - 32) >> 2) + 1, grpn);
     // the last block of the upper half ends at step 32 ceil(lt/32) + 94
     const int nsteps = 32 * ((lt + 31) >> 5) + 95;
-    for (int t = 0; t < nsteps; t++) {
+    const int tlast = lt + 94;
+    for (int t = t0; t < nsteps; t = t < tlast ? t + 1 : (t < nsteps - 1 ? nsteps - 1 : nsteps)) {
       const int c = t - lane - 32;
       const int cu = t - 32;
       if ((t & 31) == 31) {
@@ -184,9 +213,13 @@ __device__ __forceinline__ void dp_pair(const cth_dpset& q, int qp, const cth_dp
       for (int i = 0; i < CTH_NTRN; i++) nt[i] = ntn[i];
 #pragma unroll
       for (int i = 0; i < 5; i++) bcu[i] = bn[i];
-      const float scv = sc1;
-      sc1 = sc2;
-      sc2 = load_sc(c + 2);
+      const int e = (t - 32) & 3;
+      const float scv = e == 0 ? grp[0] : e == 1 ? grp[1] : e == 2 ? grp[2] : grp[3];
+      if (e == 3) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) grp[i] = grpn[i];
+        load_grp(((t - 32) >> 2) + 2, grpn);
+      }
       load_col(cu + 1, ntn, bn);
       // template transitions: lane 0 takes column t-32, the others lane-1's
 #pragma unroll
