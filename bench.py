@@ -167,7 +167,7 @@ def serve_phase(ctx, qs, db, args, seq_qps, rate=None, n=None, contraction=None,
             "mem_budget_gib": 64, "chunk_cols": 4 << 20}
 
 
-def served_burst(ctx, q, db, eval_, batch, nslots=2, bursts=4):
+def served_burst(ctx, q, db, eval_, batch, nslots=2, bursts=4, wait_us=0):
     """All of q's queries submitted at once to a serving loop over the resident
     database (budget for the whole database per request, so that a tick can
     batch them), `bursts` times on one server (its slots keep their buffers,
@@ -175,7 +175,8 @@ def served_burst(ctx, q, db, eval_, batch, nslots=2, bursts=4):
     burst is measured: GCUPS over first submit -> last done, latency
     percentiles, the largest tick."""
     import cother_amd
-    srv = cother_amd.Server(ctx, nslots=nslots, mem_budget=64 << 30, chunk_cols=0, eval=eval_, batch=batch)
+    srv = cother_amd.Server(ctx, nslots=nslots, mem_budget=64 << 30, chunk_cols=0, eval=eval_, batch=batch,
+                            batch_wait_us=wait_us)
     for _ in range(bursts):
         t0 = time.perf_counter()
         tickets = [srv.submit(qi) for qi in range(q.nprof)]
@@ -187,7 +188,8 @@ def served_burst(ctx, q, db, eval_, batch, nslots=2, bursts=4):
         wall = time.perf_counter() - t0
     srv.close()
     lat = np.array(lat)
-    return {"batch": batch, "slots": nslots, "gcups": round(float(q.lens().sum()) * db.npos / wall / 1e9, 4),
+    return {"batch": batch, "slots": nslots, "batch_wait_us": wait_us,
+            "gcups": round(float(q.lens().sum()) * db.npos / wall / 1e9, 4),
             "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
             "largest_tick": int(max(groups)), "bursts": bursts}
 
@@ -223,14 +225,17 @@ def batch_phase(args, threads, tdist):
     same = len(ref) == len(b) and all((ref[f] == b[f]).all() for f in ("query", "profile", "score", "logE", "alnlen"))
     caps, reps_g = ctx.graph_counts()
     # the serving loop on the same burst of requests: one query per tick,
-    # then ticks of up to all of them (the last of three bursts on a server)
+    # then ticks of up to all of them, started at once or after a 2 ms
+    # batching window (the last of four bursts on a server)
     served = [served_burst(ctx, q, db, args.eval, nb) for nb in (1, q.nprof)]
+    served.append(served_burst(ctx, q, db, args.eval, q.nprof, wait_us=2000))
     ctx.close()
     g1, g2 = cells * reps / per_query / 1e9, cells * reps / batched / 1e9
     log(f"batch phase (template distances {'none' if tdist == 2 else 'C-alpha'}): per-query {g1:.3f} GCUPS, "
         f"batched {g2:.3f} GCUPS ({g2 / g1:.2f}x), hits equal: {same}; served burst, one query per tick "
         f"{served[0]['gcups']:.3f} GCUPS, batched ticks {served[1]['gcups']:.3f} GCUPS "
-        f"(largest tick {served[1]['largest_tick']})")
+        f"(largest tick {served[1]['largest_tick']}), with a 2 ms batching window {served[2]['gcups']:.3f} GCUPS "
+        f"(largest tick {served[2]['largest_tick']})")
     return {"workload": "32 queries (Lq lognormal median 90, 40-150) vs 4000 templates (lognormal median 220, "
                         f"30-400), template distance lists: {'none' if tdist == 2 else 'C-alpha chains'}",
             "cells": cells, "per_query_gcups": round(g1, 4), "batched_gcups": round(g2, 4),

@@ -571,7 +571,8 @@ class Context:
 class ServerOpts(ctypes.Structure):
     """Mirror of cth_server_opts (include/cother_hip.h)."""
     _fields_ = [("nslots", ctypes.c_int), ("mem_budget", ctypes.c_uint64), ("chunk_cols", ctypes.c_uint64),
-                ("search", SearchOpts), ("alncap", ctypes.c_uint64), ("batch", ctypes.c_int)]
+                ("search", SearchOpts), ("alncap", ctypes.c_uint64), ("batch", ctypes.c_int),
+                ("batch_wait_us", ctypes.c_uint32)]
 
 
 class RequestInfo(ctypes.Structure):
@@ -585,12 +586,13 @@ class Server:
     """The serving loop (cth_server_*) over a context's uploaded queries and resident database."""
 
     def __init__(self, ctx, nslots=2, mem_budget=64 << 30, chunk_cols=4 << 20, eval=10.0, ssemodel=2, do_map=True,
-                 minpp=MINPP, alncap=0, contraction=0, batch=1):
-        """batch: requests a slot may take in one tick (run together by cth_search_batch_list)."""
+                 minpp=MINPP, alncap=0, contraction=0, batch=1, batch_wait_us=0):
+        """batch: requests a slot may take in one tick (run together by cth_search_batch_list);
+        batch_wait_us: how long a slot waits from its first request for more (0: no wait)."""
         o = ServerOpts()
         lib().cth_server_opts_init(ctypes.byref(o))
         o.nslots, o.mem_budget, o.chunk_cols, o.alncap = nslots, mem_budget, chunk_cols, alncap
-        o.batch = batch
+        o.batch, o.batch_wait_us = batch, batch_wait_us
         o.search.eval, o.search.ssemodel, o.search.do_map, o.search.minpp = eval, ssemodel, int(do_map), minpp
         o.search.contraction = contraction
         self.ctx = ctx

@@ -482,20 +482,30 @@ int search_batch_list(cth_ctx* c, const int* ql, int nq, const cth_search_opts* 
     if (!tuned) {
       // wavefronts per compute unit for the phase-1 DP: measured on 128,000
       // pairs of <= 150 x <= 400 positions (LDS for 14 per unit), 4: 47.5 ms,
-      // 6: 32.5, 8: 25.9, 10: 23.8, 12: 28.2, 14: 29.4
-      const launch_shape cand[] = {{ncu * 8, ncu * 4}, {ncu * 10, ncu * 4}, {ncu * 20, ncu * 8}};
+      // 6: 32.5, 8: 25.9, 10: 23.8, 12: 28.2, 14: 29.4.  The batch runs once
+      // with the first candidate (its results stand); then the DP's first
+      // pass alone runs again per candidate on the scores in place (it
+      // rewrites the same maxima), so that tuning a class costs one batch and
+      // a few DP passes, not a batch per candidate (with distance lists the
+      // score stage is most of a batch)
+      const int dp_cand[] = {ncu * 8, ncu * 10, ncu * 20};
       hipEvent_t e0, e1;
       CTH_HIP_CHECK(hipEventCreate(&e0));
       CTH_HIP_CHECK(hipEventCreate(&e1));
+      S = launch_shape{dp_cand[0], ncu * 4};
+      if (capture_or_launch(c, P, L, opt, ldsc, ltmax, rowcap, S, false, false)) return -1;
       float best = 1e30f;
-      for (const launch_shape& s : cand) {
+      for (const int g : dp_cand) {
+        CTH_HIP_CHECK(hipMemsetAsync(L.ctr + kCtrNext1, 0, sizeof(int), c->st));
         CTH_HIP_CHECK(hipEventRecord(e0, c->st));
-        if (capture_or_launch(c, P, L, opt, ldsc, ltmax, rowcap, s, false, false)) return -1;
+        if (cth_launch_dp_batch(&c->q.d, &db.d, L.bq, L.pairs, nullptr, nullptr, npairs, L.ctr + kCtrNext1, ltmax,
+                                ldsc, L.scores, nullptr, L.res, std::min(g, npairs), c->st))
+          return -1;
         CTH_HIP_CHECK(hipEventRecord(e1, c->st));
         CTH_HIP_CHECK(hipEventSynchronize(e1));
         float ms = 0.0f;
         CTH_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best) { best = ms; S = s; }
+        if (ms < best) { best = ms; S.dp_grid = g; }
       }
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);

@@ -14,7 +14,7 @@
 struct cth_btune {
   uint64_t key;
   int dp_grid, small_grid;
-  float ms;
+  float ms;  // the chosen phase-1 DP pass's time when tuned
 };
 
 struct cth_ctx {

@@ -20,7 +20,10 @@
 //    that fits, further waiting requests of distinct queries that fit next
 //    to it (up to batch of them), and runs them as one batched search
 //    (cth_search_batch_list: each stage once over all their pairs), so
-//    queries of mixed lengths that arrive together share launches;
+//    queries of mixed lengths that arrive together share launches; with a
+//    batching window (batch_wait_us) the slot waits that long from its first
+//    request for more, while no other slot starts a tick, so that a burst
+//    becomes one tick rather than one per idle slot;
 //  - a slot keeps its scratch between requests (freeing device memory
 //    synchronizes the device, which would stall the other slots), except
 //    that scratch beyond both its share of the budget (mem_budget / nslots)
@@ -92,6 +95,7 @@ struct cth_server {
   uint64_t inflight = 0;
   int busy = 0;
   bool stop = false;
+  bool gathering = false;  // a slot is in its batching window: the others do not start a tick
 
   uint64_t bytes_per_cell() const { return opt.search.do_map ? 13 : 9; }
   uint64_t chunk_cols() const {
@@ -199,17 +203,29 @@ struct cth_server {
       {
         std::unique_lock<std::mutex> lk(mu);
         Request* r = nullptr;
-        cv_work.wait(lk, [&] { return stop || (r = admissible(group)) != nullptr; });
+        cv_work.wait(lk, [&] { return stop || (!gathering && (r = admissible(group)) != nullptr); });
         if (!r) return;  // stopping
         // the tick: this request and, up to the batch size, the next waiting
-        // ones that fit next to it
+        // ones that fit next to it; with a batching window, the slot waits up
+        // to batch_wait_us from its first request for more to arrive, the
+        // only slot gathering meanwhile (so that a burst is not split)
+        const bool window = maxgroup > 1 && opt.batch_wait_us > 0;
+        const auto until = std::chrono::steady_clock::now() + std::chrono::microseconds(opt.batch_wait_us);
+        if (window) gathering = true;
         for (;;) {
           inflight += r->footprint;
           fp += r->footprint;
           r->slot = slot;
           r->t_start = now_ms();
           group.push_back(r);
-          if (group.size() >= maxgroup || !(r = admissible(group))) break;
+          if (group.size() >= maxgroup) break;
+          r = admissible(group);
+          if (!r && window) cv_work.wait_until(lk, until, [&] { return stop || (r = admissible(group)) != nullptr; });
+          if (!r) break;
+        }
+        if (window) {
+          gathering = false;
+          cv_work.notify_all();
         }
         busy++;
       }
@@ -248,6 +264,10 @@ extern "C" int cth_server_create(cth_ctx* parent, const cth_server_opts* o, cth_
   *out = nullptr;
   if (!parent || !o || o->nslots < 1 || o->nslots > 64) {
     cth_set_error("server: invalid arguments");
+    return -1;
+  }
+  if (o->batch_wait_us > 1000000u) {
+    cth_set_error("server: batching window of %u us (at most 1 s)", o->batch_wait_us);
     return -1;
   }
   if (cth_search_opts_check(&o->search, "server")) return -1;

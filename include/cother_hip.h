@@ -313,6 +313,8 @@ typedef struct cth_server_opts {
   cth_search_opts search;  /* per-request options (mem_budget, alnbuf, alncap are set per request) */
   uint64_t alncap;         /* per-request alignment-row capacity (0: no rows) */
   int batch;               /* requests a slot takes in one tick (<= 1: one) */
+  uint32_t batch_wait_us;  /* with batch > 1: how long a slot holding fewer than batch requests waits for
+                              more to arrive before its tick starts (0: it starts at once) */
 } cth_server_opts;
 typedef struct cth_request_info {
   int32_t qidx, slot;

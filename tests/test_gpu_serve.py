@@ -146,3 +146,34 @@ def test_batched_tick_failure_stays_per_request(ctx, sets, which):
             assert alns == wa, f"query {qi}"
     finally:
         srv.close()
+
+
+def test_batching_window_gathers_a_burst(ctx, sets):
+    """with batch_wait_us a slot that took the first request of a burst waits
+    for the rest: the burst runs as one tick, every request's results as
+    cth_search alone"""
+    q, db = sets
+    ctx.upload_queries(q)
+    ctx.upload_db(db)
+    want = [_alone(ctx, qi) for qi in range(q.nprof)]
+    srv = cother_amd.Server(ctx, nslots=1, mem_budget=1 << 34, chunk_cols=0, alncap=1 << 22, batch=q.nprof,
+                            batch_wait_us=200000)
+    try:
+        tickets = [(qi, srv.submit(qi)) for qi in range(q.nprof)]
+        groups = []
+        for qi, t in tickets:
+            hits, alns, info = srv.wait(t)
+            groups.append(info["group"])
+            wh, wa = want[qi]
+            assert len(hits) == len(wh), f"query {qi}"
+            for f in ("profile", "score", "logE", "beg", "end", "mapbeg", "mapend", "alnlen", "idnts", "psts"):
+                assert (hits[f].view(np.uint32) == wh[f].view(np.uint32)).all(), f"query {qi} field {f}"
+            assert alns == wa, f"query {qi}"
+        assert groups == [q.nprof] * q.nprof, groups
+    finally:
+        srv.close()
+
+
+def test_batching_window_over_a_second_refused(ctx, sets):
+    with pytest.raises(RuntimeError, match="batching window"):
+        cother_amd.Server(ctx, nslots=1, batch=4, batch_wait_us=2000000)
